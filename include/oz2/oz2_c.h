@@ -1,0 +1,98 @@
+/*
+ * oz2_c.h - C ABI of the MI355X (gfx950) Ozaki Scheme II DGEMM/ZGEMM library
+ * (liboz2.so).  Plain pointers and sizes only; the C++ drop-in class
+ * oz2::Ozaki2 in <oz2/ozaki2.hpp> wraps these entry points.
+ *
+ * Each entry point replaces one member of the reference's class oz2::Ozaki2
+ * (src/ozaki2_gpu.hpp, cited per function).  Matrices are column-major device
+ * pointers; complex data are interleaved (re, im) doubles.  All work is
+ * enqueued asynchronously on the handle's HIP stream.
+ *
+ * Return codes (the reference throws the matching std exception types):
+ *   OZ2_OK 0, OZ2_INVALID_ARGUMENT 1 (std::invalid_argument),
+ *   OZ2_RUNTIME_ERROR 2 (std::runtime_error), OZ2_LOGIC_ERROR 3 (std::logic_error),
+ *   OZ2_OVERFLOW_ERROR 4 (std::overflow_error).
+ * oz2_last_error() returns the message of the calling thread's last failure.
+ */
+#ifndef OZ2_C_H
+#define OZ2_C_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oz2_engine* oz2_handle;
+
+enum { OZ2_OK = 0, OZ2_INVALID_ARGUMENT = 1, OZ2_RUNTIME_ERROR = 2, OZ2_LOGIC_ERROR = 3, OZ2_OVERFLOW_ERROR = 4 };
+enum { OZ2_TRANS_N = 0, OZ2_TRANS_T = 1, OZ2_TRANS_C = 2 }; /* Trans{N,T,C}, hpp:34 */
+
+typedef struct { double scale_ms, split_ms, gemm_ms, crt_ms; } oz2_stage_times_t; /* StageTimes, hpp:36-39 */
+typedef struct { int64_t mb, nb, pm, pn, k_chunks; size_t workspace_bytes; } oz2_tile_info_t; /* TileInfo, hpp:47-52 */
+typedef struct { int batched_gemm, pad_lda, pow2_tiles, gemm_tile; } oz2_tuning_t; /* Tuning, hpp:41-45 (+ gemm_tile) */
+
+/* Ozaki2(sycl::queue&) / ~Ozaki2() (hpp:680-687).  stream: a hipStream_t, NULL = default stream. */
+int oz2_create(void* stream, oz2_handle* out);
+int oz2_destroy(oz2_handle h);
+/* rebinds the handle to another stream (no reference counterpart: a queue is fixed at construction) */
+int oz2_set_stream(oz2_handle h, void* stream);
+
+/* Ozaki2::gemm (hpp:739-760): C = alpha*op(A)*op(B) + beta*C with s = num_moduli in [2, 24].
+ * deps: ndeps hipEvent_t the stream waits for first (the reference's deps vector).
+ * done: if non-NULL, a hipEvent_t recorded after the last kernel (the returned sycl::event). */
+int oz2_dgemm(oz2_handle h, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+              int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int num_moduli,
+              void* const* deps, int ndeps, void* done);
+
+/* Ozaki2::zgemm (hpp:764-786): alpha, beta are (re, im) pairs; A, B, C interleaved complex. */
+int oz2_zgemm(oz2_handle h, int ta, int tb, int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+              int64_t lda, const double* B, int64_t ldb, const double* beta, double* C, int64_t ldc, int num_moduli,
+              void* const* deps, int ndeps, void* done);
+
+/* Ozaki2::precision_bits (hpp:691), precision_bits_complex (hpp:762), moduli_for_bits (hpp:693-700).
+ * precision_bits returns tk >= 1, or -1 with the error code in *err (may be NULL). */
+int oz2_precision_bits(int64_t k, int num_moduli, int* err);
+int oz2_precision_bits_complex(int64_t k, int num_moduli, int* err);
+int oz2_moduli_for_bits(int64_t k, int bits);
+
+/* Ozaki2::set_workspace_limit / set_resident_hint / release_workspace (hpp:702-710) */
+int oz2_set_workspace_limit(oz2_handle h, size_t bytes);
+int oz2_set_resident_hint(oz2_handle h, size_t bytes);
+int oz2_release_workspace(oz2_handle h);
+/* grows the workspace now (no reference counterpart; lets callers pre-size before graph capture) */
+int oz2_reserve_workspace(oz2_handle h, size_t bytes);
+/* bytes of workspace a call with these arguments needs (no reference counterpart) */
+int oz2_workspace_query(oz2_handle h, int ta, int tb, int64_t m, int64_t n, int64_t k, int num_moduli, int cplx,
+                        size_t* bytes);
+
+/* Ozaki2::stage_times / last_tiling / set_tuning / tuning (hpp:712-737) */
+int oz2_stage_times(oz2_handle h, oz2_stage_times_t* out);
+int oz2_last_tiling(oz2_handle h, oz2_tile_info_t* out);
+int oz2_set_tuning(oz2_handle h, const oz2_tuning_t* t);
+int oz2_get_tuning(oz2_handle h, oz2_tuning_t* out);
+/* stage timing needs a profiling queue in the reference (hpp:681); here it is a switch, default on */
+int oz2_set_profiling(oz2_handle h, int on);
+
+/* the moduli of a set (coprime_tables.hpp); writes num_moduli values, returns OZ2_INVALID_ARGUMENT if out of range */
+int oz2_moduli(int num_moduli, uint16_t* out);
+
+/* Kernel-level entry points, the counterparts of the reference's detail::launch_split and
+ * detail::launch_crt as its unit tests call them (tests/test_ozaki2.cpp:215-221, 306-313).
+ * oz2_unit_split: rows of op(X) (row r, column p at X[r*ldx + p] if kcontig, else X[r + p*ldx];
+ * complex: interleaved, 3 plane groups Re, +-Im, Re+-Im) -> int8 planes [g][t][rows_pad][kp]
+ * with exponents ex[r]; kp a multiple of 128 and >= k, rows_pad a multiple of 32 and >= rows.
+ * oz2_unit_crt: residue bytes R[t][j][ldr] (ldr a multiple of 4, >= mb) -> C[i + j*ldc] =
+ * alpha*CRT(...)*2^-(D[i]+E[j]) (+ beta*C).  Both run on `stream` and do not synchronize. */
+int oz2_unit_split(void* stream, const double* X, int64_t ldx, int64_t rows, int64_t k, int kcontig, int cplx, int conj,
+                   const int32_t* ex, int num_moduli, int tk, int8_t* planes, int64_t rows_pad, int64_t kp);
+int oz2_unit_crt(void* stream, const uint8_t* R, int64_t ldr, int64_t mb, int64_t nb, const int32_t* D,
+                 const int32_t* E, int num_moduli, double alpha, double beta, double* C, int64_t ldc);
+
+const char* oz2_last_error(void);
+const char* oz2_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -64,6 +64,9 @@ def lib():
         L.oz2o_zgemm.argtypes = [i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64, i32, i32]
         L.oz2o_dgemm_closed.argtypes = [i32, i32, i64, i64, i64, vp, i64, vp, i64, vp, i64, i32, i32]
         L.oz2o_zgemm_closed.argtypes = [i32, i32, i64, i64, i64, vp, i64, vp, i64, vp, i64, i32, i32]
+        L.oz2o_dgemm_entries.argtypes = [i32, i32, i64, i64, i64, vp, i64, vp, i64, i32, i64, vp, vp, vp]
+        L.oz2o_zgemm_entries.argtypes = [i32, i32, i64, i64, i64, vp, i64, vp, i64, i32, i64, vp, vp, vp]
+        L.oz2o_dd_entries.argtypes = [i32, i32, i64, vp, i64, vp, i64, i64, vp, vp, vp, vp, i32]
         L.oz2o_dd_gemm.argtypes = [i32, i32, i64, i64, i64, vp, i64, vp, i64, vp, vp, i32]
         L.oz2o_dd_zgemm.argtypes = [i32, i32, i64, i64, i64, vp, i64, vp, i64, vp, vp, i32]
         L.oz2o_bound_ratio.argtypes = [i32, i32, i64, i64, i64, vp, i64, vp, i64, d, d, vp, vp, i64, vp, vp, i32]
@@ -163,6 +166,28 @@ def zgemm_closed(ta, tb, m, n, k, A, lda, B, ldb, C, ldc, s, nthreads=0):
     rc = lib().oz2o_zgemm_closed(TRANS[ta], TRANS[tb], m, n, k, _p(A), lda, _p(B), ldb, _p(C), ldc, s, nthreads)
     if rc:
         raise ValueError(f"oracle closed form rejected arguments (rc={rc})")
+
+
+def gemm_entries(ta, tb, m, n, k, A, lda, B, ldb, s, rows, cols, cplx=False):
+    """Closed-form values (alpha=1, beta=0) at entries (rows[e], cols[e]); complex gives (re, im) pairs."""
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    out = np.zeros(rows.size * (2 if cplx else 1))
+    f = lib().oz2o_zgemm_entries if cplx else lib().oz2o_dgemm_entries
+    rc = f(TRANS[ta], TRANS[tb], m, n, k, _p(A), lda, _p(B), ldb, s, rows.size, _p(rows), _p(cols), _p(out))
+    if rc:
+        raise ValueError(f"oracle entries rejected arguments (rc={rc})")
+    return out
+
+
+def dd_entries(ta, tb, k, A, lda, B, ldb, rows, cols, cplx=False):
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    hi = np.zeros(rows.size * (2 if cplx else 1))
+    lo = np.zeros_like(hi)
+    lib().oz2o_dd_entries(TRANS[ta], TRANS[tb], k, _p(A), lda, _p(B), ldb, rows.size, _p(rows), _p(cols), _p(hi),
+                          _p(lo), int(cplx))
+    return hi, lo
 
 
 def dd_gemm(ta, tb, m, n, k, A, lda, B, ldb, nthreads=0):

@@ -704,6 +704,99 @@ int oz2o_zgemm_closed(int ta, int tb, int64_t m, int64_t n, int64_t k, const dou
     return 0;
 }
 
+static inline void dd_add_prod(double* sh, double* sl, double a, double b);
+
+static int32_t exponent_of_row(int trans, int is_b, int64_t i, int64_t k, const double* X, int64_t ldx, int tk,
+                               int cplx) {
+    int32_t e;
+    oz2o_exponents(trans, is_b, 1, k, is_b ? X + (trans == 0 ? i * ldx : i) * (cplx ? 2 : 1)
+                                           : X + (trans == 0 ? i : i * ldx) * (cplx ? 2 : 1),
+                   ldx, tk, cplx, &e);
+    return e;
+}
+
+int oz2o_dgemm_entries(int ta, int tb, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda, const double* B,
+                       int64_t ldb, int s, int64_t count, const int64_t* rows, const int64_t* cols, double* out) {
+    oz2o_plan p;
+    int rc = oz2o_make_plan(s, k, &p);
+    if (rc) return rc;
+    int ta_ = ta == 0 ? 0 : 1, tb_ = tb == 0 ? 0 : 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t e = 0; e < count; ++e) {
+        const int64_t i = rows[e], j = cols[e];
+        const int32_t Di = exponent_of_row(ta_, 0, i, k, A, lda, p.tk, 0);
+        const int32_t Ej = exponent_of_row(tb_, 1, j, k, B, ldb, p.tk, 0);
+        if (Di == INT_MIN || Ej == INT_MIN) { out[e] = NAN; continue; }
+        i256 acc = {{0, 0, 0, 0}};
+        for (int64_t q = 0; q < k; ++q) {
+            const sint x = scaled(A[op_index(ta_, i, q, lda)], Di), y = scaled(B[op_index(tb_, q, j, ldb)], Ej);
+            if (x.mag && y.mag) i256_add_prod(&acc, x.neg ^ y.neg, x.mag, y.mag);
+        }
+        out[e] = i256_to_double(acc, Di + Ej);
+    }
+    (void)m; (void)n;
+    return 0;
+}
+
+int oz2o_zgemm_entries(int ta, int tb, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda, const double* B,
+                       int64_t ldb, int s, int64_t count, const int64_t* rows, const int64_t* cols, double* out) {
+    oz2o_plan p;
+    int rc = oz2o_make_plan(s, 2 * k, &p);
+    if (rc) return rc;
+    int ta_ = ta == 0 ? 0 : 1, tb_ = tb == 0 ? 0 : 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t e = 0; e < count; ++e) {
+        const int64_t i = rows[e], j = cols[e];
+        const int32_t Di = exponent_of_row(ta_, 0, i, k, A, lda, p.tk, 1);
+        const int32_t Ej = exponent_of_row(tb_, 1, j, k, B, ldb, p.tk, 1);
+        if (Di == INT_MIN || Ej == INT_MIN) { out[2 * e] = NAN; out[2 * e + 1] = NAN; continue; }
+        i256 re = {{0, 0, 0, 0}}, im = {{0, 0, 0, 0}};
+        for (int64_t q = 0; q < k; ++q) {
+            const int64_t ia = op_index(ta_, i, q, lda), ib = op_index(tb_, q, j, ldb);
+            sint xr = scaled(A[2 * ia], Di), xi = scaled(A[2 * ia + 1], Di);
+            sint yr = scaled(B[2 * ib], Ej), yi = scaled(B[2 * ib + 1], Ej);
+            if (ta == 2) xi.neg ^= 1;
+            if (tb == 2) yi.neg ^= 1;
+            if (xr.mag && yr.mag) i256_add_prod(&re, xr.neg ^ yr.neg, xr.mag, yr.mag);
+            if (xi.mag && yi.mag) i256_add_prod(&re, !(xi.neg ^ yi.neg), xi.mag, yi.mag);
+            if (xr.mag && yi.mag) i256_add_prod(&im, xr.neg ^ yi.neg, xr.mag, yi.mag);
+            if (xi.mag && yr.mag) i256_add_prod(&im, xi.neg ^ yr.neg, xi.mag, yr.mag);
+        }
+        out[2 * e] = i256_to_double(re, Di + Ej);
+        out[2 * e + 1] = i256_to_double(im, Di + Ej);
+    }
+    (void)m; (void)n;
+    return 0;
+}
+
+void oz2o_dd_entries(int ta, int tb, int64_t k, const double* A, int64_t lda, const double* B, int64_t ldb,
+                     int64_t count, const int64_t* rows, const int64_t* cols, double* hi, double* lo, int cplx) {
+    int ta_ = ta == 0 ? 0 : 1, tb_ = tb == 0 ? 0 : 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t e = 0; e < count; ++e) {
+        const int64_t i = rows[e], j = cols[e];
+        if (!cplx) {
+            double sh = 0, sl = 0;
+            for (int64_t q = 0; q < k; ++q) dd_add_prod(&sh, &sl, A[op_index(ta_, i, q, lda)], B[op_index(tb_, q, j, ldb)]);
+            hi[e] = sh;
+            lo[e] = sl;
+        } else {
+            double h[2] = {0, 0}, l[2] = {0, 0};
+            for (int64_t q = 0; q < k; ++q) {
+                int64_t ia = op_index(ta_, i, q, lda), ib = op_index(tb_, q, j, ldb);
+                double ar = A[2 * ia], ai = ta == 2 ? -A[2 * ia + 1] : A[2 * ia + 1];
+                double br = B[2 * ib], bi = tb == 2 ? -B[2 * ib + 1] : B[2 * ib + 1];
+                dd_add_prod(&h[0], &l[0], ar, br);
+                dd_add_prod(&h[0], &l[0], -ai, bi);
+                dd_add_prod(&h[1], &l[1], ar, bi);
+                dd_add_prod(&h[1], &l[1], ai, br);
+            }
+            hi[2 * e] = h[0]; hi[2 * e + 1] = h[1];
+            lo[2 * e] = l[0]; lo[2 * e + 1] = l[1];
+        }
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Double-double references (test:56-84, 356-377)                            */
 /* ------------------------------------------------------------------------ */

@@ -75,6 +75,15 @@ int oz2o_dgemm_closed(int ta, int tb, int64_t m, int64_t n, int64_t k, const dou
 int oz2o_zgemm_closed(int ta, int tb, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda,
                       const double* B, int64_t ldb, double* C, int64_t ldc, int s, int nthreads);
 
+/* closed form at selected entries (rows[e], cols[e]) only: O((count + #rows + #cols) k) */
+int oz2o_dgemm_entries(int ta, int tb, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda, const double* B,
+                       int64_t ldb, int s, int64_t count, const int64_t* rows, const int64_t* cols, double* out);
+int oz2o_zgemm_entries(int ta, int tb, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda, const double* B,
+                       int64_t ldb, int s, int64_t count, const int64_t* rows, const int64_t* cols, double* out);
+/* double-double products at selected entries (hi, lo) */
+void oz2o_dd_entries(int ta, int tb, int64_t k, const double* A, int64_t lda, const double* B, int64_t ldb,
+                     int64_t count, const int64_t* rows, const int64_t* cols, double* hi, double* lo, int cplx);
+
 /* double-double references; hi/lo are m x n dense (ld m); complex: re/im interleaved pairs */
 void oz2o_dd_gemm(int ta, int tb, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda, const double* B,
                   int64_t ldb, double* hi, double* lo, int nthreads);

@@ -1,0 +1,229 @@
+// crt.hip - CRT reconstruction fused with the alpha/beta epilogue, for gfx950.
+//
+// Reference behaviour: src/ozaki2_gpu.hpp:526-664.  For each entry the s
+// residues r_t (bytes from the GEMM epilogue; k-chunks are summed mod m_t)
+// determine the exact integer C' with |C'| <= M/4:
+//   q  = rint(sum_t r_t * fp32(y_t / m_t))      (error << 1/4, so q is exact)
+//   C' = sum_t r_t W_t - q M                     (16-bit limbs, 32-bit sums)
+// C' is rounded once to 53 bits (sticky bit) and scaled by 2^-(D_i + E_j),
+// which rounds again only for subnormal results, as in the reference.
+// Rows / columns with a non-finite exponent give NaN.  The epilogue is
+// C = alpha*v (beta == 0, C not read) or fma(alpha, v, beta*C); complex:
+// Re = P0 - P1, Im = P2 - P0 - P1 in residue space (3M form), then
+// (ar*vr - ai*vi, ar*vi + ai*vr) + beta*C, each product pair fused as one fma.
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace oz2 {
+namespace k {
+using namespace dev;
+
+namespace {
+
+// exact reconstruction of one entry from its residues; NL = 32-bit words of M
+template <int NL>
+struct CrtAcc {
+    static constexpr int L = 2 * NL;
+    float u;
+    uint32_t acc[L];
+    __device__ __forceinline__ void init() {
+        u = 0.0f;
+#pragma unroll
+        for (int l = 0; l < L; ++l) acc[l] = 0;
+    }
+    __device__ __forceinline__ void add(const TabSet* __restrict__ tb, int t, uint32_t r) {
+        u = __builtin_fmaf(static_cast<float>(r), tb->frac[t], u);
+#pragma unroll
+        for (int l = 0; l < L; ++l) acc[l] += mul24(r, tb->W16[t][l]);
+    }
+    // RNE(C') * 2^-shift
+    __device__ __forceinline__ double finish(const TabSet* __restrict__ tb, int shift) const {
+        const int32_t q = static_cast<int32_t>(__builtin_rintf(u));
+        uint32_t w[NL];
+        int32_t carry = 0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const int32_t v = static_cast<int32_t>(acc[l]) - q * static_cast<int32_t>(tb->M16[l]) + carry;
+            const uint32_t lo = static_cast<uint32_t>(v) & 0xFFFFu;
+            if (l & 1) w[l >> 1] |= lo << 16;
+            else w[l >> 1] = lo;
+            carry = v >> 16;
+        }
+        const bool neg = (w[NL - 1] >> 31) != 0;
+        if (neg) {
+            uint32_t cy = 1;
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                const uint64_t s = static_cast<uint64_t>(~w[l]) + cy;
+                w[l] = static_cast<uint32_t>(s);
+                cy = static_cast<uint32_t>(s >> 32);
+            }
+        }
+        // leading nonzero word index h; window = words h, h-1, h-2 (zero below 0)
+        int h = -1;
+#pragma unroll
+        for (int l = 0; l < NL; ++l) h = w[l] ? l : h;
+        if (h < 0) return 0.0;
+        uint32_t w0 = 0, w1 = 0, w2 = 0;
+        bool below = false;
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+            w0 = (l == h) ? w[l] : w0;
+            w1 = (l == h - 1) ? w[l] : w1;
+            w2 = (l == h - 2) ? w[l] : w2;
+            below = below || (l < h - 2 && w[l] != 0);
+        }
+        const int z = __clz(static_cast<int>(w0));
+        const uint64_t hi64 = (static_cast<uint64_t>(w0) << 32) | w1;
+        uint64_t top = z ? (hi64 << z) | (w2 >> (32 - z)) : hi64;
+        const bool sticky = below || ((z ? (w2 << z) : w2) != 0);
+        top |= sticky ? 1u : 0u;
+        // top holds bits [32h + 31 - z, 32h - 32 - z]; its lsb weighs 2^(32(h-1) - z)
+        const double v = __builtin_amdgcn_ldexp(static_cast<double>(top), 32 * (h - 1) - z - shift);
+        return neg ? -v : v;
+    }
+};
+
+// residue r_t of one entry, summing k-chunks mod m_t
+__device__ __forceinline__ uint32_t chunk_sum(uint32_t s, uint32_t m) {
+    // s < kch * 256; reduce by repeated folding (kch is small)
+    while (s >= m) s -= m;
+    return s;
+}
+
+// 4 consecutive rows per thread; residues are bytes packed 4 per 32-bit load.
+template <int NL, bool CPLX>
+__global__ void __launch_bounds__(256) crt_kernel(CrtArgs a) {
+    const TabSet* __restrict__ tb = a.tab;
+    const int64_t mq = (a.mb + 3) >> 2;
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t j = idx / mq;
+    const int64_t i0 = (idx - j * mq) * 4;
+    if (j >= a.nb) return;
+    const int32_t Ej = a.E[j];
+    const uint8_t* base = a.R + j * a.ldr + i0;
+    const int s = a.s;
+
+    CrtAcc<NL> re[4];
+    CrtAcc<NL> im[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        re[e].init();
+        if (CPLX) im[e].init();
+    }
+    for (int t = 0; t < s; ++t) {
+        const uint32_t m = tb->m[t];
+        if (!CPLX) {
+            uint32_t rs[4] = {0, 0, 0, 0};
+            for (int c = 0; c < a.kch; ++c) {
+                const uint32_t w = *reinterpret_cast<const uint32_t*>(base + c * a.chunk_r + t * a.plane_r);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) rs[e] += (w >> (8 * e)) & 0xFFu;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) re[e].add(tb, t, a.kch > 1 ? chunk_sum(rs[e], m) : rs[e]);
+        } else {
+            uint32_t p0[4] = {0, 0, 0, 0}, p1[4] = {0, 0, 0, 0}, p2[4] = {0, 0, 0, 0};
+            const int64_t gs = static_cast<int64_t>(s) * a.plane_r;
+            for (int c = 0; c < a.kch; ++c) {
+                const uint8_t* b = base + c * a.chunk_r + t * a.plane_r;
+                const uint32_t w0 = *reinterpret_cast<const uint32_t*>(b);
+                const uint32_t w1 = *reinterpret_cast<const uint32_t*>(b + gs);
+                const uint32_t w2 = *reinterpret_cast<const uint32_t*>(b + 2 * gs);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    p0[e] += (w0 >> (8 * e)) & 0xFFu;
+                    p1[e] += (w1 >> (8 * e)) & 0xFFu;
+                    p2[e] += (w2 >> (8 * e)) & 0xFFu;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t x0 = a.kch > 1 ? chunk_sum(p0[e], m) : p0[e];
+                const uint32_t x1 = a.kch > 1 ? chunk_sum(p1[e], m) : p1[e];
+                const uint32_t x2 = a.kch > 1 ? chunk_sum(p2[e], m) : p2[e];
+                re[e].add(tb, t, chunk_sum(x0 + m - x1, m));               // Re = P0 - P1
+                im[e].add(tb, t, chunk_sum(x2 + 2 * m - x0 - x1, m));      // Im = P2 - P0 - P1
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int64_t i = i0 + e;
+        if (i >= a.mb) break;
+        const int32_t Di = a.D[i];
+        const bool nonfinite = Di == INT32_MIN || Ej == INT32_MIN;
+        const int shift = nonfinite ? 0 : Di + Ej;
+        if (!CPLX) {
+            double v = re[e].finish(tb, shift);
+            if (nonfinite) v = __builtin_nan("");
+            double* cp = a.C + i + j * a.ldc;
+            *cp = a.br == 0.0 ? a.ar * v : __builtin_fma(a.ar, v, a.br * *cp);
+        } else {
+            double vr = re[e].finish(tb, shift), vi = im[e].finish(tb, shift);
+            if (nonfinite) vr = vi = __builtin_nan("");
+            double* cp = a.C + 2 * (i + j * a.ldc);
+            double orr = __builtin_fma(a.ar, vr, -(a.ai * vi));
+            double oi = __builtin_fma(a.ar, vi, a.ai * vr);
+            if (a.br != 0.0 || a.bi != 0.0) {
+                const double cr = cp[0], ci = cp[1];
+                orr = orr + __builtin_fma(a.br, cr, -(a.bi * ci));
+                oi = oi + __builtin_fma(a.br, ci, a.bi * cr);
+            }
+            cp[0] = orr;
+            cp[1] = oi;
+        }
+    }
+}
+
+__global__ void scale_c_kernel(double* C, int64_t ldc, int64_t m, int64_t n, double br, double bi, int cplx) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= m * n) return;
+    const int64_t j = idx / m, i = idx - j * m;
+    if (!cplx) {
+        double* cp = C + i + j * ldc;
+        *cp = br == 0.0 ? 0.0 : br * *cp;
+    } else {
+        double* cp = C + 2 * (i + j * ldc);
+        if (br == 0.0 && bi == 0.0) {
+            cp[0] = 0.0;
+            cp[1] = 0.0;
+        } else {
+            const double r = cp[0], im = cp[1];
+            cp[0] = __builtin_fma(br, r, -(bi * im));
+            cp[1] = __builtin_fma(br, im, bi * r);
+        }
+    }
+}
+
+template <int NL>
+void launch_crt_nl(hipStream_t st, const CrtArgs& a) {
+    const int64_t threads = ((a.mb + 3) / 4) * a.nb;
+    const unsigned blocks = static_cast<unsigned>((threads + 255) / 256);
+    if (a.cplx) hipLaunchKernelGGL((crt_kernel<NL, true>), dim3(blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((crt_kernel<NL, false>), dim3(blocks), dim3(256), 0, st, a);
+}
+
+}  // namespace
+
+void crt(hipStream_t st, const CrtArgs& a) {
+    if (a.mb <= 0 || a.nb <= 0) return;
+    switch (a.L16 / 2) {
+        case 2: launch_crt_nl<2>(st, a); break;
+        case 3: launch_crt_nl<3>(st, a); break;
+        case 4: launch_crt_nl<4>(st, a); break;
+        case 5: launch_crt_nl<5>(st, a); break;
+        case 6: launch_crt_nl<6>(st, a); break;
+        default: break;
+    }
+}
+
+void scale_c(hipStream_t st, double* C, int64_t ldc, int64_t m, int64_t n, double br, double bi, bool cplx) {
+    const int64_t total = m * n;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(scale_c_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, C, ldc, m,
+                       n, br, bi, cplx ? 1 : 0);
+}
+
+}  // namespace k
+}  // namespace oz2

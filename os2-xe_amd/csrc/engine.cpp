@@ -1,0 +1,371 @@
+// engine.cpp - host orchestrator (see engine.hpp).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <limits>
+#include <stdexcept>
+#include <string>
+
+namespace oz2 {
+
+namespace {
+inline int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("oz2: ") + what + ": " + hipGetErrorString(e));
+}
+
+// MI355X cost-model constants for the tiling planner (measured on one GPU:
+// INT8 MFMA 32x32x32 4.9 Pop/s, streaming read 5.6 TB/s)
+constexpr double kInt8Rate = 4.0e15;
+constexpr double kHbmRate = 5.0e12;
+constexpr double kLaunch = 6e-6;
+}  // namespace
+
+struct Engine::Layout {
+    int64_t T = 256, kp = 0, mpad = 0, npad = 0, kch = 1;
+    int g = 1, planes = 1;
+    int64_t planeA = 0, planeB = 0, planeR = 0;
+    size_t oD = 0, oE = 0, oPA = 0, oPB = 0, oA = 0, oB = 0, oR = 0, total = 0;
+};
+
+Engine::Engine(hipStream_t stream) : stream_(stream) {
+    std::vector<TabSet> host(kMaxModuli + 1);
+    make_tabsets(host.data());
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, host.size() * sizeof(TabSet)), "table allocation");
+    dtabs_ = static_cast<TabSet*>(p);
+    const hipError_t e = hipMemcpy(dtabs_, host.data(), host.size() * sizeof(TabSet), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(dtabs_);
+        hip_check(e, "table upload");
+    }
+}
+
+Engine::~Engine() {
+    (void)hipStreamSynchronize(stream_);
+    release_workspace();
+    for (auto e : events_) (void)hipEventDestroy(e);
+    if (dtabs_) (void)hipFree(dtabs_);
+}
+
+void Engine::release_workspace() {
+    if (!ws_) return;
+    (void)hipStreamSynchronize(stream_);
+    (void)hipFree(ws_);
+    ws_ = nullptr;
+    ws_size_ = 0;
+}
+
+void Engine::reserve_workspace(size_t bytes) { ensure_workspace(bytes); }
+
+void Engine::ensure_workspace(size_t bytes) {
+    if (ws_size_ >= bytes) return;
+    // the old workspace may still be read by queued work (hpp:1159-1166)
+    hip_check(hipStreamSynchronize(stream_), "stream synchronize");
+    if (ws_) (void)hipFree(ws_);
+    ws_ = nullptr;
+    ws_size_ = 0;
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        throw std::runtime_error("oz2: workspace allocation failed");
+    }
+    ws_ = static_cast<std::byte*>(p);
+    ws_size_ = bytes;
+}
+
+const Plan& Engine::plan_for(int s, int64_t k) {
+    if (!have_plan_ || plan_.s != s || plan_.k != k) {
+        plan_ = make_plan(s, k);
+        have_plan_ = true;
+    }
+    return plan_;
+}
+
+void Engine::record(Stage st) {
+    if (!profiling_) return;
+    const size_t idx = stages_.size();
+    if (idx == events_.size()) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "event create");
+        events_.push_back(e);
+    }
+    hip_check(hipEventRecord(events_[idx], stream_), "event record");
+    stages_.emplace_back(st, idx);
+}
+
+StageTimes Engine::stage_times() {
+    StageTimes t;
+    if (!profiling_ || stages_.size() < 2) return t;
+    (void)hipEventSynchronize(events_[stages_.back().second]);
+    for (size_t i = 1; i < stages_.size(); ++i) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, events_[stages_[i - 1].second], events_[stages_[i].second]) != hipSuccess) {
+            (void)hipGetLastError();
+            ms = 0;
+        }
+        switch (stages_[i].first) {
+            case Stage::Begin: break;
+            case Stage::Scale: t.scale_ms += ms; break;
+            case Stage::Split: t.split_ms += ms; break;
+            case Stage::Gemm: t.gemm_ms += ms; break;
+            case Stage::Crt: t.crt_ms += ms; break;
+        }
+    }
+    return t;
+}
+
+size_t Engine::budget() const {
+    if (ws_limit_) return ws_limit_;
+    size_t free_b = 0, total = 0;
+    if (hipMemGetInfo(&free_b, &total) != hipSuccess) {
+        (void)hipGetLastError();
+        return size_t{1} << 34;
+    }
+    free_b += ws_size_;
+    const size_t reserve = (size_t{512} << 20) + total / 32;
+    size_t lim = free_b > reserve ? free_b - reserve : 0;
+    if (resident_hint_) lim = std::min(lim, total > reserve + resident_hint_ ? total - reserve - resident_hint_ : 0);
+    return lim;
+}
+
+k::Tile Engine::pick_tile(int64_t mb, int64_t nb, int planes) const {
+    if (tune_.gemm_tile == 256) return k::Tile::T256;
+    if (tune_.gemm_tile == 128) return k::Tile::T128;
+    if (tune_.gemm_tile == 64) return k::Tile::T64;
+    // largest tile that still gives every CU two workgroups
+    for (int64_t T : {256, 128}) {
+        const int64_t blocks = ((mb + T - 1) / T) * ((nb + T - 1) / T) * planes;
+        if (blocks >= 512) return T == 256 ? k::Tile::T256 : k::Tile::T128;
+    }
+    const int64_t b128 = ((mb + 127) / 128) * ((nb + 127) / 128) * planes;
+    return b128 >= 256 ? k::Tile::T128 : k::Tile::T64;
+}
+
+Engine::Layout Engine::layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch,
+                              int64_t mb, int64_t nb) const {
+    Layout L;
+    L.g = cplx ? 3 : 1;
+    L.planes = L.g * s;
+    L.kch = kch;
+    L.T = k::tile_rows(pick_tile(mb, nb, L.planes));
+    L.kp = round_up(k, kKStep);
+    L.mpad = round_up(mb, L.T);
+    L.npad = round_up(nb, L.T);
+    L.planeA = L.mpad * L.kp;
+    L.planeB = L.npad * L.kp;
+    L.planeR = L.npad * L.mpad;
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    L.oD = carve(static_cast<size_t>(m) * 4);
+    L.oE = carve(static_cast<size_t>(n) * 4);
+    L.oPA = carve(k::scale_partial_elems(m, k, ta != Op::N) * 8);
+    L.oPB = carve(k::scale_partial_elems(n, k, tb == Op::N) * 8);
+    L.oA = carve(static_cast<size_t>(L.planes) * L.planeA);
+    L.oB = carve(static_cast<size_t>(L.planes) * L.planeB);
+    L.oR = carve(static_cast<size_t>(kch) * L.planes * L.planeR);
+    L.total = off + 4096;
+    return L;
+}
+
+size_t Engine::workspace_for(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx) {
+    const int64_t kch = (round_up(k, kKStep) + kMaxKChunk - 1) / kMaxKChunk;
+    choose_tiling(ta, tb, m, n, k, s, cplx, kch);
+    return tiling_.workspace_bytes;
+}
+
+void Engine::choose_tiling(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch) {
+    const size_t limit = budget();
+    const double g = cplx ? 3.0 : 1.0;
+    auto candidates = [&](int64_t x) {
+        std::vector<int64_t> c{x};
+        for (int64_t p = 2; p <= 256; ++p) {
+            const int64_t b = round_up((x + p - 1) / p, 256);
+            if (b < x) c.push_back(b);
+        }
+        if (tune_.pow2_tiles)
+            for (int64_t b = 1024; b < x; b *= 2) c.push_back(b);
+        std::sort(c.begin(), c.end(), std::greater<>());
+        c.erase(std::unique(c.begin(), c.end()), c.end());
+        return c;
+    };
+    const auto nbs = candidates(n);
+    double best = std::numeric_limits<double>::infinity();
+    TileInfo bt;
+    for (int64_t mb : candidates(m)) {
+        const int64_t pm = (m + mb - 1) / mb;
+        for (int64_t nb : nbs) {
+            const int64_t pn = (n + nb - 1) / nb;
+            const Layout L = layout(ta, tb, m, n, k, s, cplx, kch, mb, nb);
+            if (L.total > limit) continue;
+            // padded GEMM work + HBM streams of split and CRT + launches (MI355X constants)
+            const double gemm = 2.0 * L.mpad * L.npad * L.kp * s * g * pm * pn / kInt8Rate;
+            const double splitA = g * (8.0 + s) * m * k * (pm > 1 ? pn : 1);
+            const double splitB = g * (8.0 + s) * k * n;
+            const double crt = (g * s * kch + 16.0 * (cplx ? 2 : 1)) * m * n;
+            const double t = gemm + (splitA + splitB + crt) / kHbmRate + pm * pn * (3 + kch) * kLaunch;
+            if (t < best) {
+                best = t;
+                bt = TileInfo{mb, nb, pm, pn, kch, L.total};
+            }
+            break;  // largest nb that fits for this mb
+        }
+    }
+    if (!std::isfinite(best))
+        throw std::runtime_error("oz2: not enough device memory for workspace (limit " + std::to_string(limit >> 20) +
+                                 " MiB)");
+    tiling_ = bt;
+}
+
+void Engine::gemm(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+                  const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int s) {
+    run(ta, tb, m, n, k, alpha, 0.0, A, lda, B, ldb, beta, 0.0, C, ldc, s, false);
+}
+
+void Engine::zgemm(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, double ai, const double* A, int64_t lda,
+                   const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s) {
+    run(ta, tb, m, n, k, ar, ai, A, lda, B, ldb, br, bi, C, ldc, s, true);
+}
+
+void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, double ai, const double* A, int64_t lda,
+                 const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s, bool cplx) {
+    stages_.clear();
+    // validation (hpp:745-748)
+    if (m < 0 || n < 0 || k < 0) throw std::invalid_argument("oz2: negative dimension");
+    if (lda < std::max<int64_t>(1, ta == Op::N ? m : k) || ldb < std::max<int64_t>(1, tb == Op::N ? k : n) ||
+        ldc < std::max<int64_t>(1, m))
+        throw std::invalid_argument("oz2: bad leading dimension");
+    if (m == 0 || n == 0) return;
+    if (k == 0 || (ar == 0.0 && ai == 0.0)) {  // C = beta * C (hpp:750-751, 1056-1070)
+        k::scale_c(stream_, C, ldc, m, n, br, bi, cplx);
+        hip_check(hipGetLastError(), "launch");
+        return;
+    }
+    const Plan& p = plan_for(s, cplx ? 2 * k : k);  // complex plan at 2k (hpp:778-781)
+    const int tk = p.tk;
+    const bool wide = tk > kNarrowScaleBits;
+    const TabSet* tab = dtabs_ + s;
+
+    const int64_t kp = round_up(k, kKStep);
+    const int64_t kch = (kp + kMaxKChunk - 1) / kMaxKChunk;
+    choose_tiling(ta, tb, m, n, k, s, cplx, kch);
+    const int64_t mb = tiling_.mb, nb = tiling_.nb;
+    const Layout L = layout(ta, tb, m, n, k, s, cplx, kch, mb, nb);
+    tile_ = pick_tile(mb, nb, L.planes);
+    ensure_workspace(L.total);
+
+    int32_t* D = reinterpret_cast<int32_t*>(ws_ + L.oD);
+    int32_t* E = reinterpret_cast<int32_t*>(ws_ + L.oE);
+    uint64_t* PA = reinterpret_cast<uint64_t*>(ws_ + L.oPA);
+    uint64_t* PB = reinterpret_cast<uint64_t*>(ws_ + L.oPB);
+    int8_t* Ap = reinterpret_cast<int8_t*>(ws_ + L.oA);
+    int8_t* Bp = reinterpret_cast<int8_t*>(ws_ + L.oB);
+    uint8_t* R = reinterpret_cast<uint8_t*>(ws_ + L.oR);
+    const int64_t cw = cplx ? 2 : 1;  // doubles per element
+
+    record(Stage::Begin);
+    // scale: exponents of the rows of op(A) and the columns of op(B) (hpp:974-989)
+    k::scale(stream_, A, lda, m, k, ta != Op::N, cplx, tk, PA, D);
+    k::scale(stream_, B, ldb, n, k, tb == Op::N, cplx, tk, PB, E);
+    record(Stage::Scale);
+
+    for (int64_t j0 = 0; j0 < n; j0 += nb) {
+        const int64_t nbj = std::min(nb, n - j0);
+        k::SplitArgs sb{};
+        sb.kcontig = tb == Op::N;
+        sb.X = sb.kcontig ? B + cw * j0 * ldb : B + cw * j0;
+        sb.ldx = ldb;
+        sb.rows = nbj;
+        sb.rows_pad = round_up(nbj, L.T);
+        sb.k = k;
+        sb.kp = kp;
+        sb.ex = E + j0;
+        sb.P = Bp;
+        sb.plane = L.planeB;
+        sb.group = static_cast<int64_t>(s) * L.planeB;
+        sb.s = s;
+        sb.cplx = cplx;
+        sb.conj = cplx && tb == Op::C;
+        sb.wide = wide;
+        sb.tab = tab;
+        k::split(stream_, sb);
+        record(Stage::Split);
+        for (int64_t i0 = 0; i0 < m; i0 += mb) {
+            const int64_t mbi = std::min(mb, m - i0);
+            if (tiling_.pm > 1 || j0 == 0) {
+                k::SplitArgs sa{};
+                sa.kcontig = ta != Op::N;
+                sa.X = sa.kcontig ? A + cw * i0 * lda : A + cw * i0;
+                sa.ldx = lda;
+                sa.rows = mbi;
+                sa.rows_pad = round_up(mbi, L.T);
+                sa.k = k;
+                sa.kp = kp;
+                sa.ex = D + i0;
+                sa.P = Ap;
+                sa.plane = L.planeA;
+                sa.group = static_cast<int64_t>(s) * L.planeA;
+                sa.s = s;
+                sa.cplx = cplx;
+                sa.conj = cplx && ta == Op::C;
+                sa.wide = wide;
+                sa.tab = tab;
+                k::split(stream_, sa);
+                record(Stage::Split);
+            }
+            k::GemmArgs ga{};
+            ga.A = Ap;
+            ga.B = Bp;
+            ga.kp = kp;
+            ga.plane_a = L.planeA;
+            ga.plane_b = L.planeB;
+            ga.plane_r = L.planeR;
+            ga.ldr = L.mpad;
+            ga.tiles_m = static_cast<int>((mbi + L.T - 1) / L.T);
+            ga.tiles_n = static_cast<int>((nbj + L.T - 1) / L.T);
+            ga.nplanes = L.planes;
+            ga.s = s;
+            ga.tab = tab;
+            for (int64_t c = 0; c < kch; ++c) {
+                ga.k0 = c * kMaxKChunk;
+                ga.klen = std::min(kMaxKChunk, kp - ga.k0);
+                ga.R = R + c * L.planes * L.planeR;
+                k::igemm_residue(stream_, tile_, ga);
+            }
+            record(Stage::Gemm);
+            k::CrtArgs ca{};
+            ca.R = R;
+            ca.plane_r = L.planeR;
+            ca.chunk_r = static_cast<int64_t>(L.planes) * L.planeR;
+            ca.kch = static_cast<int>(kch);
+            ca.ldr = L.mpad;
+            ca.mb = mbi;
+            ca.nb = nbj;
+            ca.D = D + i0;
+            ca.E = E + j0;
+            ca.C = C + cw * (i0 + j0 * ldc);
+            ca.ldc = ldc;
+            ca.ar = ar;
+            ca.ai = ai;
+            ca.br = br;
+            ca.bi = bi;
+            ca.cplx = cplx;
+            ca.s = s;
+            ca.L16 = 2 * p.NL;
+            ca.tab = tab;
+            k::crt(stream_, ca);
+            record(Stage::Crt);
+        }
+    }
+    hip_check(hipGetLastError(), "kernel launch");
+}
+
+}  // namespace oz2

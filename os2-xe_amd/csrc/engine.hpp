@@ -1,0 +1,101 @@
+// engine.hpp - host orchestrator of the Ozaki Scheme II DGEMM/ZGEMM path.
+//
+// Counterpart of the reference's class oz2::Ozaki2 (src/ozaki2_gpu.hpp:678-1188):
+// argument validation, plan cache, tiling / k-chunking, grow-only workspace,
+// stage timing, and the launch sequence scale -> split -> INT8 GEMM -> CRT on
+// one HIP stream.  All work is asynchronous on the caller's stream; the host
+// blocks only to grow the workspace and in stage_times().
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <utility>
+#include <vector>
+
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace oz2 {
+
+struct StageTimes {
+    double scale_ms = 0, split_ms = 0, gemm_ms = 0, crt_ms = 0;
+};
+
+struct TileInfo {
+    int64_t mb = 0, nb = 0;
+    int64_t pm = 0, pn = 0;
+    int64_t k_chunks = 0;
+    size_t workspace_bytes = 0;
+};
+
+struct Tuning {
+    bool batched_gemm = true;  // one launch for all planes (the only form here)
+    bool pad_lda = true;       // accepted for API parity; plane rows are padded to the GEMM tile anyway
+    bool pow2_tiles = true;    // include power-of-two tile candidates
+    int gemm_tile = -1;        // -1 auto, else 256 / 128 / 64
+};
+
+enum class Op { N = 0, T = 1, C = 2 };
+
+class Engine {
+public:
+    explicit Engine(hipStream_t stream);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    void gemm(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+              const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int s);
+    // complex: interleaved (re, im) doubles; alpha/beta as (re, im)
+    void zgemm(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, double ai, const double* A, int64_t lda,
+               const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s);
+
+    void set_workspace_limit(size_t b) { ws_limit_ = b; }
+    void set_resident_hint(size_t b) { resident_hint_ = b; }
+    void release_workspace();
+    void reserve_workspace(size_t bytes);
+    size_t workspace_size() const { return ws_size_; }
+    const TileInfo& last_tiling() const { return tiling_; }
+    void set_tuning(const Tuning& t) { tune_ = t; }
+    const Tuning& tuning() const { return tune_; }
+    void set_profiling(bool on) { profiling_ = on; }
+    bool profiling() const { return profiling_; }
+    StageTimes stage_times();
+    hipStream_t stream() const { return stream_; }
+    void set_stream(hipStream_t s) { stream_ = s; }
+
+    // Workspace a call would need (no allocation); used to pre-size before graph capture.
+    size_t workspace_for(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx);
+
+private:
+    enum class Stage { Begin, Scale, Split, Gemm, Crt };
+    void run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, double ai, const double* A, int64_t lda,
+             const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s, bool cplx);
+    const Plan& plan_for(int s, int64_t k);
+    void record(Stage st);
+    size_t budget() const;
+    k::Tile pick_tile(int64_t mb, int64_t nb, int planes) const;
+    struct Layout;
+    Layout layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch, int64_t mb,
+                  int64_t nb) const;
+    void choose_tiling(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch);
+    void ensure_workspace(size_t bytes);
+
+    hipStream_t stream_;
+    TabSet* dtabs_ = nullptr;  // device copy of make_tabsets(), s = 0..24
+    Tuning tune_{};
+    bool profiling_ = true;
+    bool have_plan_ = false;
+    Plan plan_{};
+    std::byte* ws_ = nullptr;
+    size_t ws_size_ = 0;
+    size_t ws_limit_ = 0;
+    size_t resident_hint_ = 0;
+    TileInfo tiling_{};
+    k::Tile tile_ = k::Tile::T256;
+    std::vector<hipEvent_t> events_;
+    std::vector<std::pair<Stage, size_t>> stages_;
+};
+
+}  // namespace oz2

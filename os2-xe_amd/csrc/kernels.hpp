@@ -1,0 +1,85 @@
+// kernels.hpp - host-side launch interface of the gfx950 kernels.
+//
+// Data layout in HBM (all owned by the engine's workspace):
+//   planes   int8  [group][t][row][kp]   row of op(A) (or column of op(B)) with k contiguous,
+//                                         rows padded with zeros to the GEMM tile, k to kp
+//   residues uint8 [chunk][group][t][j][ldr]  GEMM results reduced mod m_t, i contiguous
+//   D, E     int32 exponents per row of op(A) / column of op(B)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "plan.hpp"
+
+namespace oz2 {
+namespace k {
+
+// ---- scaling -----------------------------------------------------------------
+// Exponents of the `len` rows of op(X) over k columns.  kcontig: row r of op(X)
+// is contiguous (X[p + r*ldx]); otherwise rows are strided (X[r + p*ldx]).
+// Complex data (interleaved re, im) scales by the max over both parts.
+// `partial` needs partial_elems(len, k) u64 entries when !kcontig.
+size_t scale_partial_elems(int64_t len, int64_t k, bool kcontig);
+void scale(hipStream_t st, const double* X, int64_t ldx, int64_t len, int64_t k, bool kcontig, bool cplx, int tk,
+           uint64_t* partial, int32_t* ex);
+
+// ---- split -------------------------------------------------------------------
+struct SplitArgs {
+    const double* X;      // op(X) rows start at row r0 (already offset)
+    int64_t ldx;
+    int64_t rows;         // valid rows of this block
+    int64_t rows_pad;     // rows written (>= rows, multiple of 32)
+    int64_t k;            // valid k
+    int64_t kp;           // padded k (multiple of 128)
+    const int32_t* ex;    // exponent per row (already offset)
+    int8_t* P;            // planes
+    int64_t plane;        // bytes per plane (rows_pad * kp)
+    int64_t group;        // bytes per group of s planes (complex)
+    int s;
+    bool kcontig, cplx, conj, wide;
+    const TabSet* tab;    // device tables of this s
+};
+void split(hipStream_t st, const SplitArgs& a);
+
+// ---- INT8 GEMM with residue epilogue ------------------------------------------
+struct GemmArgs {
+    const int8_t* A;      // planes of op(A): [nplanes][mpad][kp]
+    const int8_t* B;      // planes of op(B): [nplanes][npad][kp]
+    uint8_t* R;           // residues [nplanes][npad_c][ldr]
+    int64_t kp;           // row stride of planes (bytes)
+    int64_t plane_a, plane_b, plane_r;
+    int64_t ldr;          // column stride of residues (= mpad)
+    int64_t k0, klen;     // k range (multiples of 128)
+    int tiles_m, tiles_n;
+    int nplanes;          // s or 3s
+    int s;
+    const TabSet* tab;    // device tables of this s
+};
+enum class Tile { T256 = 0, T128 = 1, T64 = 2 };
+int tile_rows(Tile t);
+void igemm_residue(hipStream_t st, Tile tile, const GemmArgs& a);
+
+// ---- CRT + alpha/beta epilogue -------------------------------------------------
+struct CrtArgs {
+    const uint8_t* R;     // residues of this tile
+    int64_t plane_r;      // bytes per (plane) residue matrix
+    int64_t chunk_r;      // bytes per k-chunk (nplanes * plane_r)
+    int kch;
+    int64_t ldr;
+    int64_t mb, nb;       // tile extent
+    const int32_t* D;     // per row (offset to tile)
+    const int32_t* E;     // per column (offset to tile)
+    double* C;            // offset to tile; complex: interleaved
+    int64_t ldc;
+    double ar, ai, br, bi;
+    bool cplx;
+    int s, L16;
+    const TabSet* tab;    // device tables of this s
+};
+void crt(hipStream_t st, const CrtArgs& a);
+
+// ---- C = beta * C for k == 0 or alpha == 0 --------------------------------------
+void scale_c(hipStream_t st, double* C, int64_t ldc, int64_t m, int64_t n, double br, double bi, bool cplx);
+
+}  // namespace k
+}  // namespace oz2

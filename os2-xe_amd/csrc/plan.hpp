@@ -1,0 +1,87 @@
+// plan.hpp - host-side number theory of Ozaki Scheme II: moduli sets, the
+// scaling width tk, CRT weights and the per-launch kernel tables.
+//
+// Follows the reference's make_plan / make_split_tab / make_crt_tab
+// (src/ozaki2_gpu.hpp:55-236); the moduli are the sets of
+// src/coprime_tables.hpp:10-111 for s = 2..24.
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace oz2 {
+
+inline constexpr int kMaxModuli = 24;
+inline constexpr int kMinModuli = 2;
+inline constexpr int kMaxLimbs = 6;           // 32-bit words of M for s = 24
+inline constexpr int kMaxScaleBits = 100;     // tk cap (hpp:29)
+inline constexpr int kNarrowScaleBits = 62;   // tk <= 62: |x| < 2^63 (hpp:30)
+inline constexpr int32_t kNonFinite = INT32_MIN;
+// Largest k-chunk whose int32 accumulation of int8 products is exact:
+// |sum| <= kc * 128 * 128 < 2^31.  The reference uses 131071 (hpp:32); the GEMM
+// here steps k in multiples of 128, so its chunk is the largest multiple of 128
+// below that bound.
+inline constexpr int64_t kRefMaxKChunk = 131071;
+inline constexpr int64_t kKStep = 128;
+inline constexpr int64_t kMaxKChunk = (kRefMaxKChunk / kKStep) * kKStep;  // 130944
+
+// 32-bit words of M per s: ceil-free form of the reference's limbs_for (hpp:55-59)
+constexpr int limbs_for(int s) { return s <= 8 ? 2 : s <= 12 ? 3 : s <= 16 ? 4 : s <= 20 ? 5 : 6; }
+
+const uint16_t* moduli(int s);  // s entries, or nullptr
+
+// Unsigned fixed-width integer, 5 x 64-bit little-endian limbs (320 bits).
+struct Wide {
+    static constexpr int N = 5;
+    uint64_t v[N] = {};
+    static Wide of(uint64_t x) { Wide w; w.v[0] = x; return w; }
+    void mul(uint32_t f);                 // throws std::overflow_error
+    uint32_t divmod(uint32_t d);          // *this /= d, returns remainder
+    uint32_t mod(uint32_t d) const;
+    int bits() const;                     // bit length
+    bool le(const Wide& o) const;
+    Wide shl(int n) const;                // throws std::overflow_error
+    double log2_estimate() const;         // top 96 bits as double, same value as the reference's Big::log2
+    uint32_t word32(int i) const { return static_cast<uint32_t>(v[i / 2] >> (32 * (i % 2))); }
+};
+
+struct Plan {
+    int s = 0;
+    int64_t k = 0;
+    int tk = 0;
+    int NL = 0;
+    double log2M = 0;
+    uint32_t m[kMaxModuli] = {};
+    uint32_t y[kMaxModuli] = {};
+    Wide W[kMaxModuli];
+    Wide M;
+};
+
+// Throws std::invalid_argument (s out of range, k < 1, s too small for k),
+// std::runtime_error (bad table) or std::logic_error (limb count).
+Plan make_plan(int s, int64_t k);
+
+// ---- constant tables read by the kernels -----------------------------------
+// One TabSet per s (2..24), built on the host and copied to device memory once
+// per engine; kernels read it through a const __restrict__ pointer, so the
+// per-modulus constants come in through scalar loads.
+struct TabSet {
+    int32_t s;
+    int32_t L16;                            // 16-bit limbs of M: 2 * NL
+    // split (hpp:202-220): Barrett constants and symmetric offsets
+    uint32_t m[kMaxModuli];
+    uint32_t mu[kMaxModuli];                // floor(2^32 / m)
+    uint32_t h[kMaxModuli];                 // (m - 1) / 2
+    uint32_t c16[kMaxModuli];               // 2^16 mod m
+    uint32_t c21[kMaxModuli][4];            // 2^(21 j) mod m, j = 1..4
+    uint32_t off[2][kMaxModuli];            // (m - 2^OFF mod m + h) mod m, OFF = 62 (narrow), 104 (wide)
+    uint32_t k31[kMaxModuli];               // (m - 2^31 mod m) mod m (GEMM residue epilogue)
+    // CRT (hpp:222-236)
+    float frac[kMaxModuli];                 // fp32(y_t / m_t)
+    uint32_t W16[kMaxModuli][2 * kMaxLimbs];  // 16-bit limbs of W_t
+    uint32_t M16[2 * kMaxLimbs];            // 16-bit limbs of M
+};
+// tables for s = 0..24 (entries 0 and 1 unused); y_t and W_t do not depend on k
+void make_tabsets(TabSet* out /* [kMaxModuli + 1] */);
+
+}  // namespace oz2

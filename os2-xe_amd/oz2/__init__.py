@@ -1,0 +1,270 @@
+"""Python binding of liboz2.so, the MI355X Ozaki Scheme II DGEMM/ZGEMM library.
+
+Mirrors the reference's C++ class oz2::Ozaki2 (src/ozaki2_gpu.hpp:678-786)
+over the C ABI in include/oz2/oz2_c.h: same member names, argument meaning
+and error classes (std::invalid_argument -> OZ2InvalidArgument, a ValueError;
+std::runtime_error -> OZ2RuntimeError; ...).  Matrices are column-major
+device buffers passed as torch tensors or raw device addresses (int).
+
+There is no CPU fallback: if liboz2.so is missing or cannot be loaded,
+importing this package raises.
+"""
+import ctypes
+import os
+
+try:  # load torch's HIP runtime first so liboz2.so binds to the same one
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - the library itself does not need torch
+    torch = None
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liboz2.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"oz2: {LIB_PATH} is not built (run __graft_entry__.build() or make -C os2-xe_amd)")
+_lib = ctypes.CDLL(LIB_PATH)
+
+N, T, C = 0, 1, 2
+_TRANS = {"N": 0, "T": 1, "C": 2, 0: 0, 1: 1, 2: 2}
+
+_i64, _int, _dbl, _vp, _sz = ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
+
+
+class StageTimes(ctypes.Structure):
+    _fields_ = [("scale_ms", _dbl), ("split_ms", _dbl), ("gemm_ms", _dbl), ("crt_ms", _dbl)]
+
+    def total_ms(self):
+        return self.scale_ms + self.split_ms + self.gemm_ms + self.crt_ms
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class TileInfo(ctypes.Structure):
+    _fields_ = [("mb", _i64), ("nb", _i64), ("pm", _i64), ("pn", _i64), ("k_chunks", _i64),
+                ("workspace_bytes", _sz)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("batched_gemm", _int), ("pad_lda", _int), ("pow2_tiles", _int), ("gemm_tile", _int)]
+
+
+class OZ2Error(Exception):
+    pass
+
+
+class OZ2InvalidArgument(OZ2Error, ValueError):
+    pass
+
+
+class OZ2RuntimeError(OZ2Error, RuntimeError):
+    pass
+
+
+class OZ2LogicError(OZ2Error, RuntimeError):
+    pass
+
+
+class OZ2OverflowError(OZ2Error, OverflowError):
+    pass
+
+
+_ERRORS = {1: OZ2InvalidArgument, 2: OZ2RuntimeError, 3: OZ2LogicError, 4: OZ2OverflowError}
+
+_P = ctypes.POINTER
+_lib.oz2_create.argtypes = [_vp, _P(_vp)]
+_lib.oz2_destroy.argtypes = [_vp]
+_lib.oz2_set_stream.argtypes = [_vp, _vp]
+_lib.oz2_dgemm.argtypes = [_vp, _int, _int, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl, _vp, _i64, _int,
+                           _P(_vp), _int, _vp]
+_lib.oz2_zgemm.argtypes = [_vp, _int, _int, _i64, _i64, _i64, _P(_dbl), _vp, _i64, _vp, _i64, _P(_dbl), _vp, _i64,
+                           _int, _P(_vp), _int, _vp]
+_lib.oz2_precision_bits.argtypes = [_i64, _int, _P(_int)]
+_lib.oz2_precision_bits_complex.argtypes = [_i64, _int, _P(_int)]
+_lib.oz2_moduli_for_bits.argtypes = [_i64, _int]
+_lib.oz2_set_workspace_limit.argtypes = [_vp, _sz]
+_lib.oz2_set_resident_hint.argtypes = [_vp, _sz]
+_lib.oz2_release_workspace.argtypes = [_vp]
+_lib.oz2_reserve_workspace.argtypes = [_vp, _sz]
+_lib.oz2_workspace_query.argtypes = [_vp, _int, _int, _i64, _i64, _i64, _int, _int, _P(_sz)]
+_lib.oz2_stage_times.argtypes = [_vp, _P(StageTimes)]
+_lib.oz2_last_tiling.argtypes = [_vp, _P(TileInfo)]
+_lib.oz2_set_tuning.argtypes = [_vp, _P(Tuning)]
+_lib.oz2_get_tuning.argtypes = [_vp, _P(Tuning)]
+_lib.oz2_set_profiling.argtypes = [_vp, _int]
+_lib.oz2_moduli.argtypes = [_int, _P(ctypes.c_uint16)]
+_lib.oz2_unit_split.argtypes = [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp, _int, _int, _vp, _i64, _i64]
+_lib.oz2_unit_crt.argtypes = [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _int, _dbl, _dbl, _vp, _i64]
+_lib.oz2_last_error.restype = ctypes.c_char_p
+_lib.oz2_version.restype = ctypes.c_char_p
+
+EXPORTS = [
+    "oz2_create", "oz2_destroy", "oz2_set_stream", "oz2_dgemm", "oz2_zgemm", "oz2_precision_bits",
+    "oz2_precision_bits_complex", "oz2_moduli_for_bits", "oz2_set_workspace_limit", "oz2_set_resident_hint",
+    "oz2_release_workspace", "oz2_reserve_workspace", "oz2_workspace_query", "oz2_stage_times", "oz2_last_tiling",
+    "oz2_set_tuning", "oz2_get_tuning", "oz2_set_profiling", "oz2_moduli", "oz2_unit_split", "oz2_unit_crt",
+    "oz2_last_error", "oz2_version",
+]
+
+
+def lib():
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = _lib.oz2_last_error().decode()
+        raise _ERRORS.get(rc, OZ2Error)(msg)
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if torch is not None and isinstance(x, torch.Tensor):
+        if not x.is_cuda:
+            raise OZ2InvalidArgument("oz2: tensors must live on the GPU")
+        return x.data_ptr()
+    raise TypeError(f"oz2: expected a device tensor or address, got {type(x)}")
+
+
+def moduli(num_moduli):
+    out = (ctypes.c_uint16 * 24)()
+    _check(_lib.oz2_moduli(num_moduli, out))
+    return [int(out[i]) for i in range(num_moduli)]
+
+
+def unit_split(X, ldx, rows, k, kcontig, ex, num_moduli, tk, planes, rows_pad, kp, cplx=False, conj=False,
+               stream=None):
+    """Kernel-level split (the reference's detail::launch_split in its unit test)."""
+    _check(_lib.oz2_unit_split(stream, _ptr(X), ldx, rows, k, int(kcontig), int(cplx), int(conj), _ptr(ex), num_moduli,
+                               tk, _ptr(planes), rows_pad, kp))
+
+
+def unit_crt(R, ldr, mb, nb, D, E, num_moduli, alpha, beta, C, ldc, stream=None):
+    """Kernel-level CRT + epilogue (the reference's detail::launch_crt in its unit test)."""
+    _check(_lib.oz2_unit_crt(stream, _ptr(R), ldr, mb, nb, _ptr(D), _ptr(E), num_moduli, float(alpha), float(beta),
+                             _ptr(C), ldc))
+
+
+def version():
+    return _lib.oz2_version().decode()
+
+
+class Ozaki2:
+    """oz2::Ozaki2 bound to a HIP stream (torch stream, raw hipStream_t address, or None = default)."""
+
+    def __init__(self, stream=None):
+        self._h = _vp()
+        _check(_lib.oz2_create(self._stream_ptr(stream), ctypes.byref(self._h)))
+
+    @staticmethod
+    def _stream_ptr(stream):
+        if stream is None:
+            return None
+        if isinstance(stream, int):
+            return stream
+        return stream.cuda_stream  # torch.cuda.Stream
+
+    def close(self):
+        if self._h:
+            _lib.oz2_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h.value
+
+    def set_stream(self, stream):
+        _check(_lib.oz2_set_stream(self._h, self._stream_ptr(stream)))
+
+    # -- the two GEMMs ----------------------------------------------------------------
+    def gemm(self, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, num_moduli, deps=(), done=None):
+        dep = (_vp * max(1, len(deps)))(*[int(d) for d in deps])
+        _check(_lib.oz2_dgemm(self._h, _TRANS[ta], _TRANS[tb], m, n, k, float(alpha), _ptr(A), lda, _ptr(B), ldb,
+                              float(beta), _ptr(C), ldc, num_moduli, dep, len(deps), done))
+
+    def zgemm(self, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, num_moduli, deps=(), done=None):
+        al = (_dbl * 2)(complex(alpha).real, complex(alpha).imag)
+        be = (_dbl * 2)(complex(beta).real, complex(beta).imag)
+        dep = (_vp * max(1, len(deps)))(*[int(d) for d in deps])
+        _check(_lib.oz2_zgemm(self._h, _TRANS[ta], _TRANS[tb], m, n, k, al, _ptr(A), lda, _ptr(B), ldb, be, _ptr(C),
+                              ldc, num_moduli, dep, len(deps), done))
+
+    # -- plan queries -------------------------------------------------------------------
+    @staticmethod
+    def precision_bits(k, num_moduli):
+        err = _int(0)
+        tk = _lib.oz2_precision_bits(k, num_moduli, ctypes.byref(err))
+        _check(err.value)
+        return tk
+
+    @staticmethod
+    def precision_bits_complex(k, num_moduli):
+        err = _int(0)
+        tk = _lib.oz2_precision_bits_complex(k, num_moduli, ctypes.byref(err))
+        _check(err.value)
+        return tk
+
+    @staticmethod
+    def moduli_for_bits(k, bits):
+        return _lib.oz2_moduli_for_bits(k, bits)
+
+    # -- memory ---------------------------------------------------------------------------
+    def set_workspace_limit(self, nbytes):
+        _check(_lib.oz2_set_workspace_limit(self._h, nbytes))
+
+    def set_resident_hint(self, nbytes):
+        _check(_lib.oz2_set_resident_hint(self._h, nbytes))
+
+    def release_workspace(self):
+        _check(_lib.oz2_release_workspace(self._h))
+
+    def reserve_workspace(self, nbytes):
+        _check(_lib.oz2_reserve_workspace(self._h, nbytes))
+
+    def workspace_query(self, ta, tb, m, n, k, num_moduli, cplx=False):
+        out = _sz(0)
+        _check(_lib.oz2_workspace_query(self._h, _TRANS[ta], _TRANS[tb], m, n, k, num_moduli, int(cplx),
+                                        ctypes.byref(out)))
+        return out.value
+
+    # -- introspection -------------------------------------------------------------------
+    def stage_times(self):
+        st = StageTimes()
+        _check(_lib.oz2_stage_times(self._h, ctypes.byref(st)))
+        return st
+
+    def last_tiling(self):
+        ti = TileInfo()
+        _check(_lib.oz2_last_tiling(self._h, ctypes.byref(ti)))
+        return ti
+
+    def set_tuning(self, batched_gemm=True, pad_lda=True, pow2_tiles=True, gemm_tile=-1):
+        t = Tuning(int(batched_gemm), int(pad_lda), int(pow2_tiles), int(gemm_tile))
+        _check(_lib.oz2_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self):
+        t = Tuning()
+        _check(_lib.oz2_get_tuning(self._h, ctypes.byref(t)))
+        return {"batched_gemm": bool(t.batched_gemm), "pad_lda": bool(t.pad_lda), "pow2_tiles": bool(t.pow2_tiles),
+                "gemm_tile": t.gemm_tile}
+
+    def set_profiling(self, on):
+        _check(_lib.oz2_set_profiling(self._h, int(bool(on))))
