@@ -74,6 +74,11 @@ int oz2_get_tuning(oz2_handle h, oz2_tuning_t* out);
 /* stage timing needs a profiling queue in the reference (hpp:681); here it is a switch, default on */
 int oz2_set_profiling(oz2_handle h, int on);
 
+/* stage times summed over every call between the two (no reference counterpart; the
+ * reference reports only the last call) */
+int oz2_profile_begin(oz2_handle h);
+int oz2_profile_end(oz2_handle h, oz2_stage_times_t* sum, int64_t* calls);
+
 /* the moduli of a set (coprime_tables.hpp); writes num_moduli values, returns OZ2_INVALID_ARGUMENT if out of range */
 int oz2_moduli(int num_moduli, uint16_t* out);
 

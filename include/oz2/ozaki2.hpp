@@ -1,0 +1,170 @@
+// ozaki2.hpp - drop-in C++ interface of the MI355X Ozaki Scheme II DGEMM/ZGEMM
+// library, mirroring the reference's class oz2::Ozaki2
+// (src/ozaki2_gpu.hpp:26-52 types, 678-786 members).  Header-only over the C
+// ABI in oz2_c.h (link with liboz2.so).  A sycl::queue becomes a hipStream_t;
+// the returned sycl::event becomes oz2::Event (a hipEvent_t with wait()).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <complex>
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "oz2_c.h"
+
+namespace oz2 {
+
+inline constexpr int kMaxModuli = 24;
+inline constexpr int kMaxLimbs = 6;
+inline constexpr int kMaxScaleBits = 100;
+inline constexpr int kNarrowScaleBits = 62;
+inline constexpr int kNonFinite = std::numeric_limits<int>::min();
+inline constexpr int64_t kMaxKChunk = 131071;
+
+enum class Trans { N, T, C };
+
+struct StageTimes {
+    double scale_ms = 0, split_ms = 0, gemm_ms = 0, crt_ms = 0;
+    double total_ms() const { return scale_ms + split_ms + gemm_ms + crt_ms; }
+};
+
+struct Tuning {
+    bool batched_gemm = true;
+    bool pad_lda = true;
+    bool pow2_tiles = true;
+    int gemm_tile = -1;  // MI355X extension: -1 auto, 256 / 128 / 64 forces the INT8 GEMM macro tile
+};
+
+struct TileInfo {
+    int64_t mb = 0, nb = 0;
+    int64_t pm = 0, pn = 0;
+    int64_t k_chunks = 0;
+    size_t workspace_bytes = 0;
+};
+
+namespace detail {
+[[noreturn]] inline void raise(int rc) {
+    const std::string msg = oz2_last_error();
+    switch (rc) {
+        case OZ2_INVALID_ARGUMENT: throw std::invalid_argument(msg);
+        case OZ2_LOGIC_ERROR: throw std::logic_error(msg);
+        case OZ2_OVERFLOW_ERROR: throw std::overflow_error(msg);
+        default: throw std::runtime_error(msg);
+    }
+}
+inline void check(int rc) {
+    if (rc != OZ2_OK) raise(rc);
+}
+inline int code(Trans t) { return t == Trans::N ? OZ2_TRANS_N : t == Trans::T ? OZ2_TRANS_T : OZ2_TRANS_C; }
+}  // namespace detail
+
+// Completion of one gemm/zgemm call (the reference returns a sycl::event).
+class Event {
+public:
+    Event() = default;
+    void wait() const {
+        if (ev_ && hipEventSynchronize(*ev_) != hipSuccess) throw std::runtime_error("oz2: hipEventSynchronize failed");
+    }
+    hipEvent_t native() const { return ev_ ? *ev_ : nullptr; }
+    static Event make() {
+        Event e;
+        hipEvent_t h;
+        if (hipEventCreateWithFlags(&h, hipEventDisableTiming) != hipSuccess)
+            throw std::runtime_error("oz2: hipEventCreate failed");
+        e.ev_ = std::shared_ptr<hipEvent_t>(new hipEvent_t(h), [](hipEvent_t* p) {
+            (void)hipEventDestroy(*p);
+            delete p;
+        });
+        return e;
+    }
+
+private:
+    std::shared_ptr<hipEvent_t> ev_;
+};
+
+class Ozaki2 {
+public:
+    explicit Ozaki2(hipStream_t stream = nullptr) { detail::check(oz2_create(stream, &h_)); }
+    ~Ozaki2() {
+        if (h_) oz2_destroy(h_);
+    }
+    Ozaki2(const Ozaki2&) = delete;
+    Ozaki2& operator=(const Ozaki2&) = delete;
+
+    static int precision_bits(int64_t k, int num_moduli) {
+        int err = 0;
+        const int tk = oz2_precision_bits(k, num_moduli, &err);
+        detail::check(err);
+        return tk;
+    }
+    static int precision_bits_complex(int64_t k, int num_moduli) {
+        int err = 0;
+        const int tk = oz2_precision_bits_complex(k, num_moduli, &err);
+        detail::check(err);
+        return tk;
+    }
+    static int moduli_for_bits(int64_t k, int bits) { return oz2_moduli_for_bits(k, bits); }
+
+    void set_workspace_limit(size_t bytes) { detail::check(oz2_set_workspace_limit(h_, bytes)); }
+    void set_resident_hint(size_t bytes) { detail::check(oz2_set_resident_hint(h_, bytes)); }
+    void release_workspace() { detail::check(oz2_release_workspace(h_)); }
+
+    const TileInfo& last_tiling() {
+        oz2_tile_info_t t;
+        detail::check(oz2_last_tiling(h_, &t));
+        tiling_ = TileInfo{t.mb, t.nb, t.pm, t.pn, t.k_chunks, t.workspace_bytes};
+        return tiling_;
+    }
+    void set_tuning(const Tuning& t) {
+        const oz2_tuning_t c{t.batched_gemm, t.pad_lda, t.pow2_tiles, t.gemm_tile};
+        detail::check(oz2_set_tuning(h_, &c));
+        tune_ = t;
+    }
+    const Tuning& tuning() const { return tune_; }
+
+    StageTimes stage_times() {
+        oz2_stage_times_t t;
+        detail::check(oz2_stage_times(h_, &t));
+        return StageTimes{t.scale_ms, t.split_ms, t.gemm_ms, t.crt_ms};
+    }
+
+    Event gemm(Trans ta, Trans tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+               const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int num_moduli,
+               const std::vector<Event>& deps = {}) {
+        std::vector<void*> d;
+        for (const auto& e : deps) d.push_back(e.native());
+        Event done = Event::make();
+        detail::check(oz2_dgemm(h_, detail::code(ta), detail::code(tb), m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                                num_moduli, d.data(), static_cast<int>(d.size()), done.native()));
+        return done;
+    }
+
+    Event zgemm(Trans ta, Trans tb, int64_t m, int64_t n, int64_t k, std::complex<double> alpha,
+                const std::complex<double>* A, int64_t lda, const std::complex<double>* B, int64_t ldb,
+                std::complex<double> beta, std::complex<double>* C, int64_t ldc, int num_moduli,
+                const std::vector<Event>& deps = {}) {
+        std::vector<void*> d;
+        for (const auto& e : deps) d.push_back(e.native());
+        Event done = Event::make();
+        const double al[2] = {alpha.real(), alpha.imag()}, be[2] = {beta.real(), beta.imag()};
+        detail::check(oz2_zgemm(h_, detail::code(ta), detail::code(tb), m, n, k, al,
+                                reinterpret_cast<const double*>(A), lda, reinterpret_cast<const double*>(B), ldb, be,
+                                reinterpret_cast<double*>(C), ldc, num_moduli, d.data(), static_cast<int>(d.size()),
+                                done.native()));
+        return done;
+    }
+
+    oz2_handle native_handle() const { return h_; }
+
+private:
+    oz2_handle h_ = nullptr;
+    Tuning tune_{};
+    TileInfo tiling_{};
+};
+
+}  // namespace oz2

@@ -233,6 +233,19 @@ int oz2_set_profiling(oz2_handle h, int on) {
     return guard([&] { need(h); h->eng.set_profiling(on != 0); });
 }
 
+int oz2_profile_begin(oz2_handle h) {
+    return guard([&] { need(h); h->eng.profile_begin(); });
+}
+
+int oz2_profile_end(oz2_handle h, oz2_stage_times_t* sum, int64_t* calls) {
+    return guard([&] {
+        need(h);
+        if (!sum) throw std::invalid_argument("oz2: null output pointer");
+        const oz2::StageTimes t = h->eng.profile_end(calls);
+        *sum = oz2_stage_times_t{t.scale_ms, t.split_ms, t.gemm_ms, t.crt_ms};
+    });
+}
+
 int oz2_moduli(int num_moduli, uint16_t* out) {
     return guard([&] {
         const uint16_t* ms = oz2::moduli(num_moduli);

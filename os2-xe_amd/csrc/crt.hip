@@ -11,78 +11,14 @@
 // C = alpha*v (beta == 0, C not read) or fma(alpha, v, beta*C); complex:
 // Re = P0 - P1, Im = P2 - P0 - P1 in residue space (3M form), then
 // (ar*vr - ai*vi, ar*vi + ai*vr) + beta*C, each product pair fused as one fma.
-#include "device.hpp"
 #include "kernels.hpp"
+#include "math.hpp"
 
 namespace oz2 {
 namespace k {
-using namespace dev;
+using namespace math;
 
 namespace {
-
-// exact reconstruction of one entry from its residues; NL = 32-bit words of M
-template <int NL>
-struct CrtAcc {
-    static constexpr int L = 2 * NL;
-    float u;
-    uint32_t acc[L];
-    __device__ __forceinline__ void init() {
-        u = 0.0f;
-#pragma unroll
-        for (int l = 0; l < L; ++l) acc[l] = 0;
-    }
-    __device__ __forceinline__ void add(const TabSet* __restrict__ tb, int t, uint32_t r) {
-        u = __builtin_fmaf(static_cast<float>(r), tb->frac[t], u);
-#pragma unroll
-        for (int l = 0; l < L; ++l) acc[l] += mul24(r, tb->W16[t][l]);
-    }
-    // RNE(C') * 2^-shift
-    __device__ __forceinline__ double finish(const TabSet* __restrict__ tb, int shift) const {
-        const int32_t q = static_cast<int32_t>(__builtin_rintf(u));
-        uint32_t w[NL];
-        int32_t carry = 0;
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-            const int32_t v = static_cast<int32_t>(acc[l]) - q * static_cast<int32_t>(tb->M16[l]) + carry;
-            const uint32_t lo = static_cast<uint32_t>(v) & 0xFFFFu;
-            if (l & 1) w[l >> 1] |= lo << 16;
-            else w[l >> 1] = lo;
-            carry = v >> 16;
-        }
-        const bool neg = (w[NL - 1] >> 31) != 0;
-        if (neg) {
-            uint32_t cy = 1;
-#pragma unroll
-            for (int l = 0; l < NL; ++l) {
-                const uint64_t s = static_cast<uint64_t>(~w[l]) + cy;
-                w[l] = static_cast<uint32_t>(s);
-                cy = static_cast<uint32_t>(s >> 32);
-            }
-        }
-        // leading nonzero word index h; window = words h, h-1, h-2 (zero below 0)
-        int h = -1;
-#pragma unroll
-        for (int l = 0; l < NL; ++l) h = w[l] ? l : h;
-        if (h < 0) return 0.0;
-        uint32_t w0 = 0, w1 = 0, w2 = 0;
-        bool below = false;
-#pragma unroll
-        for (int l = 0; l < NL; ++l) {
-            w0 = (l == h) ? w[l] : w0;
-            w1 = (l == h - 1) ? w[l] : w1;
-            w2 = (l == h - 2) ? w[l] : w2;
-            below = below || (l < h - 2 && w[l] != 0);
-        }
-        const int z = __clz(static_cast<int>(w0));
-        const uint64_t hi64 = (static_cast<uint64_t>(w0) << 32) | w1;
-        uint64_t top = z ? (hi64 << z) | (w2 >> (32 - z)) : hi64;
-        const bool sticky = below || ((z ? (w2 << z) : w2) != 0);
-        top |= sticky ? 1u : 0u;
-        // top holds bits [32h + 31 - z, 32h - 32 - z]; its lsb weighs 2^(32(h-1) - z)
-        const double v = __builtin_amdgcn_ldexp(static_cast<double>(top), 32 * (h - 1) - z - shift);
-        return neg ? -v : v;
-    }
-};
 
 // residue r_t of one entry, summing k-chunks mod m_t
 __device__ __forceinline__ uint32_t chunk_sum(uint32_t s, uint32_t m) {

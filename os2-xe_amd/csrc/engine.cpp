@@ -119,6 +119,18 @@ StageTimes Engine::stage_times() {
     return t;
 }
 
+void Engine::profile_begin() {
+    stages_.clear();
+    calls_ = 0;
+    accumulate_ = true;
+}
+
+StageTimes Engine::profile_end(int64_t* calls) {
+    accumulate_ = false;
+    if (calls) *calls = calls_;
+    return stage_times();
+}
+
 size_t Engine::budget() const {
     if (ws_limit_) return ws_limit_;
     size_t free_b = 0, total = 0;
@@ -237,7 +249,8 @@ void Engine::zgemm(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, dou
 
 void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, double ai, const double* A, int64_t lda,
                  const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s, bool cplx) {
-    stages_.clear();
+    if (!accumulate_) stages_.clear();
+    ++calls_;
     // validation (hpp:745-748)
     if (m < 0 || n < 0 || k < 0) throw std::invalid_argument("oz2: negative dimension");
     if (lda < std::max<int64_t>(1, ta == Op::N ? m : k) || ldb < std::max<int64_t>(1, tb == Op::N ? k : n) ||

@@ -62,6 +62,9 @@ public:
     void set_profiling(bool on) { profiling_ = on; }
     bool profiling() const { return profiling_; }
     StageTimes stage_times();
+    // stage times summed over every call between profile_begin() and profile_end()
+    void profile_begin();
+    StageTimes profile_end(int64_t* calls);
     hipStream_t stream() const { return stream_; }
     void set_stream(hipStream_t s) { stream_ = s; }
 
@@ -96,6 +99,8 @@ private:
     k::Tile tile_ = k::Tile::T256;
     std::vector<hipEvent_t> events_;
     std::vector<std::pair<Stage, size_t>> stages_;
+    bool accumulate_ = false;
+    int64_t calls_ = 0;
 };
 
 }  // namespace oz2

@@ -17,12 +17,15 @@
 // MFMA 32x32x32 i8 operand layout (checked on gfx950): lane l holds row
 // (l mod 32) of A / column (l mod 32) of B with 16 k-bytes at k = 16*(l/32);
 // accumulator v of lane l is C[8*(v/4) + 4*(l/32) + v%4][l mod 32].
-#include "device.hpp"
 #include "kernels.hpp"
+#include "math.hpp"
 
 namespace oz2 {
 namespace k {
-using namespace dev;
+using namespace math;
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 
 namespace {
 
@@ -145,7 +148,6 @@ __global__ void __launch_bounds__(256, 1) igemm_residue_kernel(GemmArgs p) {
     // epilogue: int32 -> residue mod m_t, 4 rows packed per 32-bit store
     const int t = static_cast<int>(plane % static_cast<uint32_t>(p.s));
     const TabSet* __restrict__ tb = p.tab;
-    const uint32_t m = tb->m[t], mu = tb->mu[t], c16 = tb->c16[t], k31 = tb->k31[t];
     uint8_t* Rb = p.R + static_cast<int64_t>(plane) * p.plane_r;
     const int64_t col0 = static_cast<int64_t>(tn) * BN + wn * WTN + fr;
     const int64_t row0 = static_cast<int64_t>(tm) * BM + wm * WTM + 4 * fc;
@@ -158,16 +160,7 @@ __global__ void __launch_bounds__(256, 1) igemm_residue_kernel(GemmArgs p) {
                 uint32_t w = 0;
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    const uint32_t x = static_cast<uint32_t>(acc[i][j][4 * q + b]);
-                    uint32_t r;
-                    if (m == 256) {
-                        r = x & 0xFFu;
-                    } else {
-                        // (x + 2^31) mod m, then remove 2^31 mod m again: result in [0, m)
-                        r = mod32(x ^ 0x80000000u, m, mu, c16) + k31;
-                        r = min(r, r - m);
-                    }
-                    w |= r << (8 * b);
+                    w |= acc_residue(static_cast<uint32_t>(acc[i][j][4 * q + b]), tb, t) << (8 * b);
                 }
                 *reinterpret_cast<uint32_t*>(Rb + (col0 + j * 32) * p.ldr + row0 + i * 32 + 8 * q) = w;
             }

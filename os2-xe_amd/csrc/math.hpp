@@ -1,0 +1,226 @@
+// math.hpp - per-element integer arithmetic of the Ozaki Scheme II path,
+// shared by the gfx950 kernels and by the host build used to test it on the
+// CPU (csrc/test/emu.cpp).  Everything here is exact integer work except the
+// fp32 CRT quotient estimate, whose error is bounded far below its margin.
+#pragma once
+#include <cstdint>
+
+#include "plan.hpp"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define OZ2_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define OZ2_HD inline
+#endif
+
+namespace oz2 {
+namespace math {
+
+// v_mul_u32_u24 / v_mul_hi_u32_u24: low 32 and high 16 bits of a 24 x 24-bit product
+OZ2_HD uint32_t mul24(uint32_t a, uint32_t b) {
+    return static_cast<uint32_t>(static_cast<uint64_t>(a & 0xFFFFFFu) * static_cast<uint64_t>(b & 0xFFFFFFu));
+}
+OZ2_HD uint32_t mulhi24(uint32_t a, uint32_t b) {
+    return static_cast<uint32_t>((static_cast<uint64_t>(a & 0xFFFFFFu) * static_cast<uint64_t>(b & 0xFFFFFFu)) >> 32);
+}
+OZ2_HD uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+// x mod m for x < 2^24, 2 <= m <= 255, mu = floor(2^31 / m) (< 2^24 since m >= 129).
+// Barrett with q = floor(x mu / 2^31): q is floor(x/m) or one less, so r = x - q m
+// lies in [0, 2m) and one conditional subtraction finishes.
+OZ2_HD uint32_t mod24(uint32_t x, uint32_t m, uint32_t mu) {
+    const uint64_t prod = (static_cast<uint64_t>(mulhi24(x, mu)) << 32) | mul24(x, mu);  // 48-bit x*mu
+    const uint32_t q = static_cast<uint32_t>(prod >> 31);
+    const uint32_t r = x - mul24(q, m);
+    return umin(r, r - m);
+}
+
+// x mod m for any 32-bit x (m <= 255): fold the high half with c16 = 2^16 mod m,
+// (x >> 16) c16 + (x & 0xffff) <= 65535 * 254 + 65535 < 2^24, then Barrett.
+OZ2_HD uint32_t mod32(uint32_t x, uint32_t m, uint32_t mu, uint32_t c16) {
+    return mod24(mul24(x >> 16, c16) + (x & 0xFFFFu), m, mu);
+}
+
+OZ2_HD int clz32(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+OZ2_HD int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
+
+// frexp exponent of a positive finite double given by its bit pattern (bits != 0)
+OZ2_HD int frexp_exp_bits(uint64_t b) {
+    const int bexp = static_cast<int>(b >> 52);
+    if (bexp) return bexp - 1022;
+    return (64 - clz64(b & 0xFFFFFFFFFFFFFull)) - 1074;
+}
+
+// exponent rule of the scaling step (hpp:360-377): 0 for an all-zero row,
+// kNonFinite for Inf/NaN (bit patterns >= the infinity pattern), else tk - frexp_exp(max)
+OZ2_HD int32_t exponent_from_maxbits(uint64_t b, int tk) {
+    if (b >= 0x7FF0000000000000ull) return INT32_MIN;
+    if (b == 0) return 0;
+    return tk - frexp_exp_bits(b);
+}
+
+// Exact RNE(|v| * 2^e) for |v * 2^e| < 2^101 as a 128-bit magnitude (hpp:251-289).
+OZ2_HD void scaled_mag(uint64_t bits, int e, uint64_t& hi, uint64_t& lo) {
+    const int bexp = static_cast<int>((bits >> 52) & 0x7FF);
+    uint64_t mant = bits & 0xFFFFFFFFFFFFFull;
+    if (bexp) mant |= 0x10000000000000ull;
+    const int sh = (bexp ? bexp : 1) - 1075 + e;
+    if (sh >= 0) {
+        if (sh >= 64) {
+            hi = mant << (sh - 64);
+            lo = 0;
+        } else {
+            lo = mant << sh;
+            hi = sh == 0 ? 0 : (mant >> (64 - sh));
+        }
+    } else {
+        const int rs = -sh < 63 ? -sh : 63;
+        const uint64_t q = mant >> rs;
+        const uint64_t half = 1ull << (rs - 1);
+        const uint64_t rem = mant & ((half << 1) - 1);
+        lo = q + ((rem > half || (rem == half && (q & 1))) ? 1 : 0);
+        hi = 0;
+    }
+}
+
+// Offset-binary u = 2^OFF + x (OFF = 62 narrow, 104 wide; x = +-RNE(|v| 2^e)) as
+// 21-bit limbs a[0..4].  zero: the element is outside the matrix or its row is
+// non-finite, giving x = 0.
+template <bool WIDE>
+OZ2_HD void to_limbs(uint64_t bits, int e, bool zero, uint32_t (&a)[5]) {
+    uint64_t hi = 0, lo = 0;
+    if (!zero) scaled_mag(bits, e, hi, lo);
+    const bool neg = (bits >> 63) != 0 && !zero;
+    uint64_t uh, ul;
+    if (WIDE) {  // mag <= 2^100
+        if (neg) {
+            ul = 0 - lo;
+            uh = (1ull << 40) - hi - (lo != 0 ? 1 : 0);
+        } else {
+            ul = lo;
+            uh = (1ull << 40) + hi;
+        }
+    } else {  // mag <= 2^62
+        uh = 0;
+        ul = neg ? (1ull << 62) - lo : (1ull << 62) + lo;
+    }
+    constexpr uint64_t M21 = 0x1FFFFF;
+    a[0] = static_cast<uint32_t>(ul & M21);
+    a[1] = static_cast<uint32_t>((ul >> 21) & M21);
+    if (WIDE) {
+        a[2] = static_cast<uint32_t>((ul >> 42) & M21);
+        a[3] = static_cast<uint32_t>(((ul >> 63) | (uh << 1)) & M21);
+        a[4] = static_cast<uint32_t>((uh >> 20) & M21);
+    } else {
+        a[2] = static_cast<uint32_t>(ul >> 42);  // u <= 2^63: at most 22 bits
+        a[3] = a[4] = 0;
+    }
+}
+
+// (x + h) mod m in [0, m) for modulus index t (m <= 255) from the limbs of u = 2^OFF + x:
+// sum a_j (2^(21 j) mod m) + (h - 2^OFF mod m) stays below 2^32, then mod32.
+template <bool WIDE>
+OZ2_HD uint32_t limb_residue(const uint32_t (&a)[5], const TabSet* __restrict__ tb, int t) {
+    uint32_t v = a[0] + tb->off[WIDE ? 1 : 0][t];
+    v += mul24(a[1], tb->c21[t][0]);
+    v += mul24(a[2], tb->c21[t][1]);
+    if (WIDE) {
+        v += mul24(a[3], tb->c21[t][2]);
+        v += mul24(a[4], tb->c21[t][3]);
+    }
+    return mod32(v, tb->m[t], tb->mu[t], tb->c16[t]);
+}
+
+// int8 byte of the symmetric residue of x for modulus index t (m = 256 only at t = 0)
+template <bool WIDE>
+OZ2_HD uint32_t sym_byte(const uint32_t (&a)[5], const TabSet* __restrict__ tb, int t) {
+    if (t == 0) return a[0] & 0xFFu;  // m = 256: x mod 256 = u mod 256
+    return (limb_residue<WIDE>(a, tb, t) - tb->h[t]) & 0xFFu;
+}
+
+// symmetric residue value in [-h, m-1-h] (complex split needs the value, not the byte)
+template <bool WIDE>
+OZ2_HD int32_t sym_value(const uint32_t (&a)[5], const TabSet* __restrict__ tb, int t) {
+    if (t == 0) return static_cast<int32_t>((a[0] + 127u) & 0xFFu) - 127;
+    return static_cast<int32_t>(limb_residue<WIDE>(a, tb, t)) - static_cast<int32_t>(tb->h[t]);
+}
+
+// GEMM epilogue: int32 dot product -> residue in [0, m)
+OZ2_HD uint32_t acc_residue(uint32_t x, const TabSet* __restrict__ tb, int t) {
+    const uint32_t m = tb->m[t];
+    if (m == 256) return x & 0xFFu;
+    uint32_t r = mod32(x ^ 0x80000000u, m, tb->mu[t], tb->c16[t]) + tb->k31[t];  // (x + 2^31) - 2^31 mod m
+    return umin(r, r - m);
+}
+
+OZ2_HD double ldexp_(double v, int e) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ldexp(v, e);
+#else
+    return std::ldexp(v, e);
+#endif
+}
+
+// Exact reconstruction of one entry from residues r_t in [0, m_t); NL = 32-bit words of M.
+template <int NL>
+struct CrtAcc {
+    static constexpr int L = 2 * NL;
+    float u;
+    uint32_t acc[L];
+    OZ2_HD void init() {
+        u = 0.0f;
+        for (int l = 0; l < L; ++l) acc[l] = 0;
+    }
+    // u += r y_t/m_t (fp32); acc_l += r W16[t][l]  (r < 256, W16 < 2^16: products < 2^24, sums < 2^29)
+    OZ2_HD void add(const TabSet* __restrict__ tb, int t, uint32_t r) {
+        u = __builtin_fmaf(static_cast<float>(r), tb->frac[t], u);
+        for (int l = 0; l < L; ++l) acc[l] += mul24(r, tb->W16[t][l]);
+    }
+    // RNE(C') * 2^-shift with C' = sum r_t W_t - q M, q = rint(u)
+    OZ2_HD double finish(const TabSet* __restrict__ tb, int shift) const {
+        const int32_t q = static_cast<int32_t>(__builtin_rintf(u));
+        uint32_t w[NL];
+        int32_t carry = 0;
+        for (int l = 0; l < L; ++l) {
+            const int32_t v = static_cast<int32_t>(acc[l]) - q * static_cast<int32_t>(tb->M16[l]) + carry;
+            const uint32_t lo = static_cast<uint32_t>(v) & 0xFFFFu;
+            if (l & 1) w[l >> 1] |= lo << 16;
+            else w[l >> 1] = lo;
+            carry = v >> 16;
+        }
+        const bool neg = (w[NL - 1] >> 31) != 0;
+        if (neg) {
+            uint32_t cy = 1;
+            for (int l = 0; l < NL; ++l) {
+                const uint64_t s = static_cast<uint64_t>(~w[l]) + cy;
+                w[l] = static_cast<uint32_t>(s);
+                cy = static_cast<uint32_t>(s >> 32);
+            }
+        }
+        // leading nonzero word h; 64-bit window from words h, h-1, h-2 plus a sticky bit
+        int h = -1;
+        for (int l = 0; l < NL; ++l) h = w[l] ? l : h;
+        if (h < 0) return 0.0;
+        uint32_t w0 = 0, w1 = 0, w2 = 0;
+        bool below = false;
+        for (int l = 0; l < NL; ++l) {
+            w0 = (l == h) ? w[l] : w0;
+            w1 = (l == h - 1) ? w[l] : w1;
+            w2 = (l == h - 2) ? w[l] : w2;
+            below = below || (l < h - 2 && w[l] != 0);
+        }
+        const int z = clz32(w0);
+        const uint64_t hi64 = (static_cast<uint64_t>(w0) << 32) | w1;
+        uint64_t top = z ? (hi64 << z) | (w2 >> (32 - z)) : hi64;
+        const bool sticky = below || ((z ? (w2 << z) : w2) != 0);
+        top |= sticky ? 1u : 0u;
+        // the window's lsb weighs 2^(32(h-1) - z); one RNE here, a second only if subnormal
+        const double v = ldexp_(static_cast<double>(top), 32 * (h - 1) - z - shift);
+        return neg ? -v : v;
+    }
+};
+
+}  // namespace math
+}  // namespace oz2

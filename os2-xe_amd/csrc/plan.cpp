@@ -157,7 +157,7 @@ void make_tabsets(TabSet* out) {
             const uint32_t m = p.m[t];
             const uint32_t h = (m - 1) / 2;
             tb.m[t] = m;
-            tb.mu[t] = static_cast<uint32_t>((uint64_t{1} << 32) / m);
+            tb.mu[t] = static_cast<uint32_t>((uint64_t{1} << 31) / m);
             tb.h[t] = h;
             tb.c16[t] = pow2_mod(16, m);
             for (int j = 1; j <= 4; ++j) tb.c21[t][j - 1] = pow2_mod(21 * j, m);

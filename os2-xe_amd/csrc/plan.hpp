@@ -70,7 +70,7 @@ struct TabSet {
     int32_t L16;                            // 16-bit limbs of M: 2 * NL
     // split (hpp:202-220): Barrett constants and symmetric offsets
     uint32_t m[kMaxModuli];
-    uint32_t mu[kMaxModuli];                // floor(2^32 / m)
+    uint32_t mu[kMaxModuli];                // floor(2^31 / m): Barrett constant for 24-bit operands
     uint32_t h[kMaxModuli];                 // (m - 1) / 2
     uint32_t c16[kMaxModuli];               // 2^16 mod m
     uint32_t c21[kMaxModuli][4];            // 2^(21 j) mod m, j = 1..4

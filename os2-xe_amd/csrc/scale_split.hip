@@ -12,12 +12,12 @@
 // with lanes along rows, so no separate transpose pass is needed.
 #include <algorithm>
 
-#include "device.hpp"
 #include "kernels.hpp"
+#include "math.hpp"
 
 namespace oz2 {
 namespace k {
-using namespace dev;
+using namespace math;
 
 namespace {
 
@@ -108,51 +108,6 @@ int64_t scale_kblocks(int64_t len, int64_t k) {
 // split
 // ---------------------------------------------------------------------------
 
-// residue of the offset-binary scaled integer u = 2^OFF + x (OFF = 62 narrow, 104 wide),
-// given as 21-bit limbs; returns (x + h) mod m in [0, m)  (m <= 255)
-template <bool WIDE>
-__device__ __forceinline__ uint32_t limb_residue(const uint32_t (&a)[5], const TabSet* __restrict__ tb, int t) {
-    uint32_t v = a[0] + tb->off[WIDE ? 1 : 0][t];
-    v += mul24(a[1], tb->c21[t][0]);
-    v += mul24(a[2], tb->c21[t][1]);
-    if (WIDE) {
-        v += mul24(a[3], tb->c21[t][2]);
-        v += mul24(a[4], tb->c21[t][3]);
-    }
-    return mod32(v, tb->m[t], tb->mu[t], tb->c16[t]);
-}
-
-template <bool WIDE>
-__device__ __forceinline__ void to_limbs(uint64_t bits, int e, bool zero, uint32_t (&a)[5]) {
-    uint64_t hi = 0, lo = 0;
-    if (!zero) scaled_mag(bits, e, hi, lo);
-    const bool neg = (bits >> 63) != 0 && !zero;
-    uint64_t uh, ul;
-    if (WIDE) {  // u = 2^104 +- mag  (mag <= 2^100)
-        if (neg) {
-            ul = 0 - lo;
-            uh = (1ull << 40) - hi - (lo != 0 ? 1 : 0);
-        } else {
-            ul = lo;
-            uh = (1ull << 40) + hi;
-        }
-    } else {  // u = 2^62 +- mag  (mag <= 2^62)
-        uh = 0;
-        ul = neg ? (1ull << 62) - lo : (1ull << 62) + lo;
-    }
-    constexpr uint64_t M21 = 0x1FFFFF;
-    a[0] = static_cast<uint32_t>(ul & M21);
-    a[1] = static_cast<uint32_t>((ul >> 21) & M21);
-    if (WIDE) {
-        a[2] = static_cast<uint32_t>((ul >> 42) & M21);
-        a[3] = static_cast<uint32_t>(((ul >> 63) | (uh << 1)) & M21);
-        a[4] = static_cast<uint32_t>((uh >> 20) & M21);
-    } else {
-        a[2] = static_cast<uint32_t>(ul >> 42);
-        a[3] = a[4] = 0;
-    }
-}
-
 // Real split: each thread owns 16 consecutive k of one row -> one 16-byte store per plane.
 // KCONTIG: row r is contiguous (X[p + r*ldx]); block = 256 chunks along the row(s).
 // !KCONTIG: rows strided (X[r + p*ldx]); block = 32 rows x 8 chunks (128 k).
@@ -188,26 +143,13 @@ __global__ void __launch_bounds__(256) split_real_kernel(SplitArgs a, int lg_tpr
         to_limbs<WIDE>(bits, e, z, limbs[q]);
     }
     int8_t* out = a.P + r * a.kp + p0;
-    // modulus 256 (always plane 0): the stored byte is x mod 256 = u mod 256
-    {
-        uint32_t w[4];
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
-            w[g] = (limbs[4 * g][0] & 0xFF) | ((limbs[4 * g + 1][0] & 0xFF) << 8) |
-                   ((limbs[4 * g + 2][0] & 0xFF) << 16) | ((limbs[4 * g + 3][0] & 0xFF) << 24);
-        *reinterpret_cast<uint4*>(out) = make_uint4(w[0], w[1], w[2], w[3]);
-    }
-    for (int t = 1; t < a.s; ++t) {
-        const uint32_t h = tb->h[t];
+    for (int t = 0; t < a.s; ++t) {
         uint32_t w[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             uint32_t acc = 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t rr = limb_residue<WIDE>(limbs[4 * g + q], tb, t) - h;
-                acc |= (rr & 0xFFu) << (8 * q);
-            }
+            for (int q = 0; q < 4; ++q) acc |= sym_byte<WIDE>(limbs[4 * g + q], tb, t) << (8 * q);
             w[g] = acc;
         }
         *reinterpret_cast<uint4*>(out + t * a.plane) = make_uint4(w[0], w[1], w[2], w[3]);
@@ -256,14 +198,8 @@ __global__ void __launch_bounds__(256) split_cplx_kernel(SplitArgs a, int lg_tpr
         uint32_t w0 = 0, w1 = 0, w2 = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            int32_t s0, s1;
-            if (t == 0) {  // m = 256, h = 127: (x + 127) mod 256 from the low byte
-                s0 = static_cast<int32_t>((lr[q][0] + 127u) & 0xFFu) - 127;
-                s1 = static_cast<int32_t>((li[q][0] + 127u) & 0xFFu) - 127;
-            } else {
-                s0 = static_cast<int32_t>(limb_residue<WIDE>(lr[q], tb, t)) - h;
-                s1 = static_cast<int32_t>(limb_residue<WIDE>(li[q], tb, t)) - h;
-            }
+            const int32_t s0 = sym_value<WIDE>(lr[q], tb, t);
+            int32_t s1 = sym_value<WIDE>(li[q], tb, t);
             if (CONJ) s1 = -s1;
             int32_t ss = s0 + s1;
             ss = ss > m - 1 - h ? ss - m : ss;

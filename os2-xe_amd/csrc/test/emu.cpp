@@ -1,0 +1,98 @@
+// emu.cpp - host build of the kernels' per-element arithmetic (csrc/math.hpp),
+// so the CPU test suite can compare it bit-for-bit with the oracle without a
+// GPU.  Test support only (lib/liboz2_emu.so); the product library is liboz2.so.
+#include <cstdint>
+#include <vector>
+
+#include "../math.hpp"
+
+using namespace oz2;
+
+namespace {
+const TabSet* tables() {
+    static std::vector<TabSet> t = [] {
+        std::vector<TabSet> v(kMaxModuli + 1);
+        make_tabsets(v.data());
+        return v;
+    }();
+    return t.data();
+}
+}  // namespace
+
+extern "C" {
+
+// symmetric int8 residues of RNE(v_i * 2^e_i): planes[t * n + i]; e = INT32_MIN gives 0
+void emu_split(const double* v, const int32_t* e, int64_t n, int s, int tk, int8_t* planes) {
+    const TabSet* tb = tables() + s;
+    const bool wide = tk > kNarrowScaleBits;
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t a[5];
+        uint64_t bits;
+        __builtin_memcpy(&bits, v + i, 8);
+        const bool zero = e[i] == INT32_MIN;
+        if (wide) math::to_limbs<true>(bits, zero ? 0 : e[i], zero, a);
+        else math::to_limbs<false>(bits, zero ? 0 : e[i], zero, a);
+        for (int t = 0; t < s; ++t)
+            planes[t * n + i] = static_cast<int8_t>(wide ? math::sym_byte<true>(a, tb, t) : math::sym_byte<false>(a, tb, t));
+    }
+}
+
+// complex: groups Re, Im (negated if conj), Re+Im reduced to [-h, m-1-h]: planes[(g*s + t)*n + i]
+void emu_split_cplx(const double* re, const double* im, const int32_t* e, int64_t n, int s, int tk, int conj,
+                    int8_t* planes) {
+    const TabSet* tb = tables() + s;
+    const bool wide = tk > kNarrowScaleBits;
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t ar[5], ai[5];
+        uint64_t br, bi;
+        __builtin_memcpy(&br, re + i, 8);
+        __builtin_memcpy(&bi, im + i, 8);
+        const bool zero = e[i] == INT32_MIN;
+        const int ee = zero ? 0 : e[i];
+        if (wide) { math::to_limbs<true>(br, ee, zero, ar); math::to_limbs<true>(bi, ee, zero, ai); }
+        else { math::to_limbs<false>(br, ee, zero, ar); math::to_limbs<false>(bi, ee, zero, ai); }
+        for (int t = 0; t < s; ++t) {
+            const int32_t m = static_cast<int32_t>(tb->m[t]), h = static_cast<int32_t>(tb->h[t]);
+            const int32_t s0 = wide ? math::sym_value<true>(ar, tb, t) : math::sym_value<false>(ar, tb, t);
+            int32_t s1 = wide ? math::sym_value<true>(ai, tb, t) : math::sym_value<false>(ai, tb, t);
+            if (conj) s1 = -s1;
+            int32_t ss = s0 + s1;
+            ss = ss > m - 1 - h ? ss - m : ss;
+            ss = ss < -h ? ss + m : ss;
+            planes[(0 * s + t) * n + i] = static_cast<int8_t>(s0);
+            planes[(1 * s + t) * n + i] = static_cast<int8_t>(s1);
+            planes[(2 * s + t) * n + i] = static_cast<int8_t>(ss);
+        }
+    }
+}
+
+// GEMM epilogue: residues of int32 sums, out[t * n + i] in [0, m_t)
+void emu_acc_residue(const int32_t* x, int64_t n, int s, uint8_t* out) {
+    const TabSet* tb = tables() + s;
+    for (int t = 0; t < s; ++t)
+        for (int64_t i = 0; i < n; ++i) out[t * n + i] = static_cast<uint8_t>(math::acc_residue(static_cast<uint32_t>(x[i]), tb, t));
+}
+
+// CRT of residues res[i * s + t] in [0, m_t): out[i] = RNE(C') * 2^-shift[i]
+void emu_crt(const uint8_t* res, const int32_t* shift, int64_t n, int s, double* out) {
+    const TabSet* tb = tables() + s;
+    const int nl = limbs_for(s);
+    for (int64_t i = 0; i < n; ++i) {
+        const uint8_t* r = res + i * s;
+        switch (nl) {
+#define OZ2_EMU_NL(NL)                                     \
+    case NL: {                                             \
+        math::CrtAcc<NL> a;                                \
+        a.init();                                          \
+        for (int t = 0; t < s; ++t) a.add(tb, t, r[t]);    \
+        out[i] = a.finish(tb, shift[i]);                   \
+        break;                                             \
+    }
+            OZ2_EMU_NL(2) OZ2_EMU_NL(3) OZ2_EMU_NL(4) OZ2_EMU_NL(5) OZ2_EMU_NL(6)
+#undef OZ2_EMU_NL
+        }
+    }
+}
+
+int emu_exponent_from_maxbits(uint64_t b, int tk) { return math::exponent_from_maxbits(b, tk); }
+}

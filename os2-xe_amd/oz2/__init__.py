@@ -96,6 +96,8 @@ _lib.oz2_set_tuning.argtypes = [_vp, _P(Tuning)]
 _lib.oz2_get_tuning.argtypes = [_vp, _P(Tuning)]
 _lib.oz2_set_profiling.argtypes = [_vp, _int]
 _lib.oz2_moduli.argtypes = [_int, _P(ctypes.c_uint16)]
+_lib.oz2_profile_begin.argtypes = [_vp]
+_lib.oz2_profile_end.argtypes = [_vp, _P(StageTimes), _P(_i64)]
 _lib.oz2_unit_split.argtypes = [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp, _int, _int, _vp, _i64, _i64]
 _lib.oz2_unit_crt.argtypes = [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _int, _dbl, _dbl, _vp, _i64]
 _lib.oz2_last_error.restype = ctypes.c_char_p
@@ -105,7 +107,8 @@ EXPORTS = [
     "oz2_create", "oz2_destroy", "oz2_set_stream", "oz2_dgemm", "oz2_zgemm", "oz2_precision_bits",
     "oz2_precision_bits_complex", "oz2_moduli_for_bits", "oz2_set_workspace_limit", "oz2_set_resident_hint",
     "oz2_release_workspace", "oz2_reserve_workspace", "oz2_workspace_query", "oz2_stage_times", "oz2_last_tiling",
-    "oz2_set_tuning", "oz2_get_tuning", "oz2_set_profiling", "oz2_moduli", "oz2_unit_split", "oz2_unit_crt",
+    "oz2_set_tuning", "oz2_get_tuning", "oz2_set_profiling", "oz2_profile_begin", "oz2_profile_end", "oz2_moduli",
+    "oz2_unit_split", "oz2_unit_crt",
     "oz2_last_error", "oz2_version",
 ]
 
@@ -265,6 +268,16 @@ class Ozaki2:
         _check(_lib.oz2_get_tuning(self._h, ctypes.byref(t)))
         return {"batched_gemm": bool(t.batched_gemm), "pad_lda": bool(t.pad_lda), "pow2_tiles": bool(t.pow2_tiles),
                 "gemm_tile": t.gemm_tile}
+
+    def profile_begin(self):
+        _check(_lib.oz2_profile_begin(self._h))
+
+    def profile_end(self):
+        """(summed StageTimes, number of calls) since profile_begin()"""
+        st = StageTimes()
+        calls = _i64(0)
+        _check(_lib.oz2_profile_end(self._h, ctypes.byref(st), ctypes.byref(calls)))
+        return st, calls.value
 
     def set_profiling(self, on):
         _check(_lib.oz2_set_profiling(self._h, int(bool(on))))

@@ -1,0 +1,136 @@
+"""CPU checks of the kernels' per-element arithmetic (os2-xe_amd/csrc/math.hpp),
+compiled for the host as lib/liboz2_emu.so, against the oracle and the golden
+vectors: the exact split (both scaling widths, complex groups), the GEMM
+residue epilogue, the CRT reconstruction and the exponent rule.  The GPU tests
+then only have to establish that the kernels apply this arithmetic to the
+right elements."""
+import ctypes
+import json
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+_emu = ctypes.CDLL(os.path.join(ROOT, "os2-xe_amd", "lib", "liboz2_emu.so"))
+vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+_emu.emu_split.argtypes = [vp, vp, i64, i32, i32, vp]
+_emu.emu_split_cplx.argtypes = [vp, vp, vp, i64, i32, i32, i32, vp]
+_emu.emu_acc_residue.argtypes = [vp, i64, i32, vp]
+_emu.emu_crt.argtypes = [vp, vp, i64, i32, vp]
+_emu.emu_exponent_from_maxbits.argtypes = [ctypes.c_uint64, i32]
+
+
+def P(a):
+    return a.ctypes.data_as(vp)
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+def h2f(h):
+    return struct.unpack("<d", struct.pack("<Q", h))[0]
+
+
+def emu_split(v, e, s, tk):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.int32)
+    out = np.zeros((s, v.size), dtype=np.int8)
+    _emu.emu_split(P(v), P(e), v.size, s, tk, P(out))
+    return out
+
+
+def test_split_golden():
+    for case in load("split.json"):
+        for col in case["cols"]:
+            v = np.array([h2f(b) for b in col["bits"]])
+            e = np.full(v.size, col["e"], dtype=np.int32)
+            assert emu_split(v, e, case["s"], case["tk"]).tolist() == col["res"], (case["s"], case["tk"])
+
+
+@pytest.mark.parametrize("s,tk", [(2, 6), (8, 24), (13, 44), (16, 55), (18, 62), (19, 63), (24, 86), (24, 100)])
+def test_split_random_vs_oracle(s, tk, rng):
+    n = 20000
+    v = rng.uniform(-1, 1, n) * np.exp2(rng.integers(-70, 1, n).astype(float))
+    v[:8] = [0.0, -0.0, 5e-324, -5e-324, 1.0 - 2 ** -53, -(1.0 - 2 ** -53), 0.5, -0.75]
+    mx = np.abs(v).max()
+    e = np.full(n, tk - math.frexp(mx)[1], dtype=np.int32)
+    e[:50] = rng.integers(tk - 60, tk + 1, 50)  # other exponents, |v 2^e| < 2^tk still
+    e[50] = -2 ** 31  # non-finite row: all residues zero
+    got = emu_split(v, e, s, tk)
+    want = O.split_values(v, np.where(e == -2 ** 31, 0, e), s)
+    want[:, 50] = 0
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("conj", [0, 1])
+@pytest.mark.parametrize("s,tk", [(14, 49), (16, 55), (24, 86)])
+def test_split_complex_vs_oracle(s, tk, conj, rng):
+    n = 5000
+    re = rng.uniform(-1, 1, n)
+    im = rng.uniform(-1, 1, n)
+    e = np.full(n, tk, dtype=np.int32)
+    out = np.zeros((3, s, n), dtype=np.int8)
+    _emu.emu_split_cplx(P(re), P(im), P(e), n, s, tk, conj, P(out))
+    r0 = O.split_values(re, e, s).astype(np.int64)
+    r1 = O.split_values(im, e, s).astype(np.int64)
+    for t, m in enumerate(O.moduli(s)):
+        h = (m - 1) // 2
+        a = np.where(r0[t] < -h, r0[t] + m, r0[t])
+        b = np.where(r1[t] < -h, r1[t] + m, r1[t])
+        if conj:
+            b = -b
+        ss = (((a + b) % m) + h) % m - h
+        for g, want in enumerate((a, b, ss)):
+            assert np.array_equal(out[g, t].astype(np.int64) & 0xFF, want & 0xFF), (t, g)
+
+
+@pytest.mark.parametrize("s", [2, 7, 14, 16, 24])
+def test_acc_residue(s, rng):
+    x = rng.integers(-2 ** 31, 2 ** 31, 100000, dtype=np.int64)
+    x[:6] = [0, -1, 1, -2 ** 31, 2 ** 31 - 1, 130944 * 2 ** 14 - 1]
+    out = np.zeros((s, x.size), dtype=np.uint8)
+    xi = x.astype(np.int32)
+    _emu.emu_acc_residue(P(xi), x.size, s, P(out))
+    for t, m in enumerate(O.moduli(s)):
+        assert np.array_equal(out[t].astype(np.int64), x % m), m
+
+
+def test_crt_golden():
+    for case in load("crt.json"):
+        s = case["s"]
+        res = np.array(case["res"], dtype=np.uint8)
+        sh = np.array(case["shift"], dtype=np.int32)
+        out = np.zeros(len(sh))
+        _emu.emu_crt(P(res), P(sh), len(sh), s, P(out))
+        assert out.view(np.uint64).tolist() == [int(x) for x in case["expect"]], s
+
+
+@pytest.mark.parametrize("s", list(range(2, 25)))
+def test_crt_random_vs_oracle(s, rng):
+    plan = O.make_plan(s, 16)
+    ms = O.moduli(s)
+    M4 = plan.M_int // 4
+    n = 2000
+    xs = [int(rng.integers(0, 2 ** 62)) * int(rng.integers(0, 2 ** 62)) * int(rng.integers(0, 2 ** 62)) for _ in range(n)]
+    xs = [(x % (M4 + 1)) * (1 if i % 3 else -1) for i, x in enumerate(xs)]
+    xs[0], xs[1], xs[2] = M4, -M4, 0
+    res = np.array([[x % m for m in ms] for x in xs], dtype=np.uint8)
+    sh = rng.integers(-40, 40, n).astype(np.int32)
+    out = np.zeros(n)
+    _emu.emu_crt(P(res), P(sh), n, s, P(out))
+    want = np.array([O.crt_value(plan, res[i], int(sh[i])) for i in range(n)])
+    assert np.array_equal(out.view(np.uint64), want.view(np.uint64))
+
+
+def test_exponent_rule():
+    for b, tk in ((0, 30), (0x7FF0000000000000, 30), (0x7FF8000000000000, 5), (1, 55), (0x3FF0000000000000, 55),
+                  (0x000FFFFFFFFFFFFF, 55), (0x7FEFFFFFFFFFFFFF, 100)):
+        assert _emu.emu_exponent_from_maxbits(b, tk) == O.exponent_from_maxbits(b, tk)
