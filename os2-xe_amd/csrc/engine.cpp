@@ -149,8 +149,12 @@ k::Tile Engine::pick_tile(int64_t mb, int64_t nb, int planes) const {
     if (tune_.gemm_tile == 256) return k::Tile::T256;
     if (tune_.gemm_tile == 128) return k::Tile::T128;
     if (tune_.gemm_tile == 64) return k::Tile::T64;
+    // never pad a small tile up to a bigger macro tile
+    const int64_t lim = std::min(mb, nb);
+    if (lim <= 64) return k::Tile::T64;
     // largest tile that still gives every CU two workgroups
     for (int64_t T : {256, 128}) {
+        if (T == 256 && lim <= 128) continue;
         const int64_t blocks = ((mb + T - 1) / T) * ((nb + T - 1) / T) * planes;
         if (blocks >= 512) return T == 256 ? k::Tile::T256 : k::Tile::T128;
     }
@@ -200,7 +204,7 @@ void Engine::choose_tiling(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s,
     auto candidates = [&](int64_t x) {
         std::vector<int64_t> c{x};
         for (int64_t p = 2; p <= 256; ++p) {
-            const int64_t b = round_up((x + p - 1) / p, 256);
+            const int64_t b = round_up((x + p - 1) / p, 64);
             if (b < x) c.push_back(b);
         }
         if (tune_.pow2_tiles)

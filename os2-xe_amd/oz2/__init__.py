@@ -200,14 +200,14 @@ class Ozaki2:
     # -- the two GEMMs ----------------------------------------------------------------
     def gemm(self, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, num_moduli, deps=(), done=None):
         dep = (_vp * max(1, len(deps)))(*[int(d) for d in deps])
-        _check(_lib.oz2_dgemm(self._h, _TRANS[ta], _TRANS[tb], m, n, k, float(alpha), _ptr(A), lda, _ptr(B), ldb,
+        _check(_lib.oz2_dgemm(self._h, _TRANS.get(ta, -1), _TRANS.get(tb, -1), m, n, k, float(alpha), _ptr(A), lda, _ptr(B), ldb,
                               float(beta), _ptr(C), ldc, num_moduli, dep, len(deps), done))
 
     def zgemm(self, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, num_moduli, deps=(), done=None):
         al = (_dbl * 2)(complex(alpha).real, complex(alpha).imag)
         be = (_dbl * 2)(complex(beta).real, complex(beta).imag)
         dep = (_vp * max(1, len(deps)))(*[int(d) for d in deps])
-        _check(_lib.oz2_zgemm(self._h, _TRANS[ta], _TRANS[tb], m, n, k, al, _ptr(A), lda, _ptr(B), ldb, be, _ptr(C),
+        _check(_lib.oz2_zgemm(self._h, _TRANS.get(ta, -1), _TRANS.get(tb, -1), m, n, k, al, _ptr(A), lda, _ptr(B), ldb, be, _ptr(C),
                               ldc, num_moduli, dep, len(deps), done))
 
     # -- plan queries -------------------------------------------------------------------
@@ -244,7 +244,7 @@ class Ozaki2:
 
     def workspace_query(self, ta, tb, m, n, k, num_moduli, cplx=False):
         out = _sz(0)
-        _check(_lib.oz2_workspace_query(self._h, _TRANS[ta], _TRANS[tb], m, n, k, num_moduli, int(cplx),
+        _check(_lib.oz2_workspace_query(self._h, _TRANS.get(ta, -1), _TRANS.get(tb, -1), m, n, k, num_moduli, int(cplx),
                                         ctypes.byref(out)))
         return out.value
 

@@ -71,3 +71,18 @@ def test_error_message_prefix():
         oz2.Ozaki2.precision_bits(100, 30)
     except oz2.OZ2InvalidArgument as e:
         assert str(e).startswith("oz2: ")
+
+
+def test_cpp_dropin_header_compiles_and_links(tmp_path):
+    """The C++ drop-in header (include/oz2/ozaki2.hpp) builds the reference-style example and links liboz2.so."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    src = os.path.join(ROOT, "os2-xe_amd", "examples", "dgemm.cpp")
+    lib = os.path.join(ROOT, "os2-xe_amd", "lib")
+    out = tmp_path / "dgemm_example"
+    r = subprocess.run([hipcc, "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-L", lib, "-loz2",
+                        "-o", str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
