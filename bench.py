@@ -77,9 +77,11 @@ def cpu_baseline(m, n, k, s):
     """Oracle (CPU restatement, OpenMP) on the first `rows` rows of op(A): same k, n, s, distribution."""
     import numpy as np
     from oracle import oracle as O
-    threads = os.cpu_count() or 1
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
     rng = np.random.default_rng(1)
-    rows = 128
+    rows = 3072  # about 10-15 s of OpenMP work on 16 host cores
     A = rng.uniform(-1, 1, rows * k)
     B = rng.uniform(-1, 1, k * n)
     C = rng.uniform(-1, 1, rows * n)

@@ -8,11 +8,23 @@
 // every entry mod m_t to a byte, the only information the CRT needs
 // (hpp:590-601), cutting the CRT's reads from 4s to s bytes per entry.
 //
+// Operand planes are stored blocked (plane_offset() in kernels.hpp): 1 KB per
+// 16 rows x 64 k, the four 16-byte k-chunks of a row XOR-permuted by
+// (row >> 2) & 3.  One stage of a macro tile is then a run of whole 1 KB
+// blocks, each fetched by one global_load_lds_dwordx4 (lane l -> bytes
+// 16l..16l+15) straight into LDS in the same order, and the MFMA operand reads
+// (lane l: row l mod 32, chunk 2 ks + l / 32) are free of bank conflicts:
+// ds_read_b128 serves 16 lanes per pass from the quads {0,3,5,6} and {1,2,4,7}
+// of each half-wave over 64 banks, and rows 4q..4q+3 of such a quad set land
+// on four distinct chunk positions.
+//
 // Kernel shape: macro tile BM x BN, 256 threads = 4 waves in 2 x 2, each wave
-// (BM/2) x (BN/2) built from v_mfma_i32_32x32x32_i8 blocks.  k advances 128
-// bytes per LDS stage, two stages double-buffered; 16-byte chunks of a row
-// are XOR-swizzled so both the 16-byte stores and the MFMA operand reads are
-// free of LDS bank conflicts.
+// (BM/2) x (BN/2) from v_mfma_i32_32x32x32_i8 blocks.  k advances 64 bytes per
+// stage; NST = 4 stages are in flight (global -> LDS by DMA, no staging
+// registers) and the operand fragments are double-buffered in registers, so
+// each half-iteration issues, in a fixed order, the MFMAs of the current
+// stage, the ds_reads of the next stage and the DMAs of stage + 3; a wave's
+// VMEM and LDS issue never sits in front of a run of its MFMAs.
 //
 // MFMA 32x32x32 i8 operand layout (checked on gfx950): lane l holds row
 // (l mod 32) of A / column (l mod 32) of B with 16 k-bytes at k = 16*(l/32);
@@ -26,12 +38,18 @@ using namespace math;
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void* lds_ptr;
+typedef __attribute__((address_space(1))) void* glb_ptr;
 
 namespace {
 
-constexpr int KT = 128;  // k bytes per LDS stage (8 chunks of 16 bytes)
+constexpr int KT = 64;   // k bytes per LDS stage = one plane block column
+constexpr int NST = 4;   // LDS stages
 
-__device__ __forceinline__ int swz(int row, int c) { return c ^ ((row ^ (row >> 3)) & 7); }
+// s_waitcnt immediate waiting for vmcnt <= n only (gfx9 encoding: vmcnt in bits
+// [3:0] and [15:14]; expcnt [6:4] and lgkmcnt [11:8] left at their maximum)
+constexpr int vm_wait(int n) { return 0x0F70 | (n & 15) | ((n >> 4) << 14); }
+constexpr int kLgkmWait0 = 0xC07F & ~0x0F00;  // lgkmcnt(0) only
 
 // blockIdx -> work item with the 8 XCDs of MI355X each given a contiguous
 // range (workgroups are dispatched round-robin over XCDs, so consecutive ids
@@ -41,13 +59,68 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
     return (b & 7) * (n >> 3) + (b >> 3);
 }
 
-template <int BM, int BN>
-__global__ void __launch_bounds__(256, 1) igemm_residue_kernel(GemmArgs p) {
+// Residues of one wave tile (WTM x WTN) staged through LDS: 4 residues per
+// 32-bit LDS store into a [col][row] byte tile whose 16-byte chunks are
+// XOR-permuted by column, then 16-byte global stores that write whole runs of
+// WTM bytes of each column of R.
+template <int FM, int FN>
+__device__ __forceinline__ void residue_epilogue(const GemmArgs& p, v16i (&acc)[FM][FN], uint32_t plane,
+                                                 int64_t row0, int64_t col0, char* wlds, int lane) {
+    constexpr int WTM = FM * 32, WTN = FN * 32, NCH = WTM / 16;
+    const int t = static_cast<int>(plane % static_cast<uint32_t>(p.s));
+    const TabSet* __restrict__ tb = p.tab;
+    const uint32_t m = tb->m[t];
+    const int fr = lane & 31, fc = lane >> 5;
+    auto lds_at = [&](int c, int r) { return wlds + c * WTM + ((((r >> 4) ^ (c & (NCH - 1))) << 4) | (r & 15)); };
+    if (m == 256) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint32_t w = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) w |= (static_cast<uint32_t>(acc[i][j][4 * q + b]) & 0xFFu) << (8 * b);
+                    *reinterpret_cast<uint32_t*>(lds_at(j * 32 + fr, i * 32 + 8 * q + 4 * fc)) = w;
+                }
+    } else {
+        const uint32_t mu = tb->mu[t], c16 = tb->c16[t], c31 = tb->c31[t];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint32_t w = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) w |= residue_m(static_cast<uint32_t>(acc[i][j][4 * q + b]), m, mu, c16, c31) << (8 * b);
+                    *reinterpret_cast<uint32_t*>(lds_at(j * 32 + fr, i * 32 + 8 * q + 4 * fc)) = w;
+                }
+    }
+    __syncthreads();
+    uint8_t* Rb = p.R + static_cast<int64_t>(plane) * p.plane_r;
+    constexpr int CPI = 64 / NCH;  // columns per pass
+#pragma unroll
+    for (int it = 0; it < WTN / CPI; ++it) {
+        const int c = it * CPI + lane / NCH, ch = lane % NCH;
+        const v4i v = *reinterpret_cast<const v4i*>(wlds + c * WTM + ((ch ^ (c & (NCH - 1))) << 4));
+        *reinterpret_cast<v4i*>(Rb + (col0 + c) * p.ldr + row0 + ch * 16) = v;
+    }
+}
+
+template <int BM, int BN, int MINB>
+__global__ void __launch_bounds__(256, MINB) igemm_residue_kernel(GemmArgs p) {
     constexpr int WTM = BM / 2, WTN = BN / 2;
     constexpr int FM = WTM / 32, FN = WTN / 32;
-    constexpr int LA = BM / 32, LB = BN / 32;
-    constexpr int STAGE = (BM + BN) * 8;  // int4 per stage
-    __shared__ v4i smem[2 * STAGE];
+    constexpr int LA = BM / 64, LB = BN / 64;  // 1 KB DMAs per wave per stage
+    constexpr int STAGE = (BM + BN) * KT;       // bytes
+    constexpr int NR = 2 * (FM + FN);           // fragment reads per stage
+    constexpr int NMF = 2 * FM * FN;            // MFMAs per stage
+    constexpr int ND = LA + LB;                 // DMAs per stage
+    constexpr int VM_AHEAD = (NST - 3) * ND;    // DMAs still in flight when a stage must have landed
+    static_assert(WTM * WTN * 4 <= NST * STAGE, "epilogue staging must fit in the stage buffers");
+    __shared__ v4i smem[NST * STAGE / 16];
 
     const uint32_t nblk = gridDim.x;
     const uint32_t bid = xcd_remap(blockIdx.x, nblk);
@@ -70,15 +143,26 @@ __global__ void __launch_bounds__(256, 1) igemm_residue_kernel(GemmArgs p) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int lrow = tid >> 3, lchk = tid & 7;
+    const int fr = lane & 31, fc = lane >> 5;
+    const int nkt = static_cast<int>(p.klen / KT);  // even: klen is a multiple of 128
+    const int64_t kblocks = p.kp / KT;              // plane blocks per 16 rows
+    // this lane's 16 bytes of each of its DMA blocks at stage 0
+    const int8_t* Ag = p.A + static_cast<int64_t>(plane) * p.plane_a +
+                       ((static_cast<int64_t>(tm) * (BM / 16) + wave * LA) * kblocks + p.k0 / KT) * 1024 + lane * 16;
+    const int8_t* Bg = p.B + static_cast<int64_t>(plane) * p.plane_b +
+                       ((static_cast<int64_t>(tn) * (BN / 16) + wave * LB) * kblocks + p.k0 / KT) * 1024 + lane * 16;
+    const int64_t bstride = kblocks * 1024;  // next 16-row block
+    char* sbase = reinterpret_cast<char*>(smem);
+    // LDS byte offsets of this lane's fragments (row-major 64-byte rows; the
+    // chunk permutation depends on row bits 2-3 only, so fragment i is +2 KB * i)
+    int offa[2], offb[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        const int ra = wm * WTM + fr, rb = wn * WTN + fr;
+        offa[ks] = ra * 64 + (((2 * ks + fc) ^ ((ra >> 2) & 3)) << 4);
+        offb[ks] = BM * KT + rb * 64 + (((2 * ks + fc) ^ ((rb >> 2) & 3)) << 4);
+    }
 
-    const int8_t* Ag = p.A + static_cast<int64_t>(plane) * p.plane_a + static_cast<int64_t>(tm) * BM * p.kp + p.k0 +
-                       static_cast<int64_t>(lrow) * p.kp + lchk * 16;
-    const int8_t* Bg = p.B + static_cast<int64_t>(plane) * p.plane_b + static_cast<int64_t>(tn) * BN * p.kp + p.k0 +
-                       static_cast<int64_t>(lrow) * p.kp + lchk * 16;
-    const int64_t rowstep = 32 * p.kp;
-
-    v4i ra[LA], rb[LB];
     v16i acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -86,84 +170,79 @@ __global__ void __launch_bounds__(256, 1) igemm_residue_kernel(GemmArgs p) {
         for (int j = 0; j < FN; ++j)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0;
+    v4i fa0[2][FM], fb0[2][FN], fa1[2][FM], fb1[2][FN];
 
-#define OZ2_GLOAD(KT_)                                                                             \
-    {                                                                                              \
-        const int64_t ko = static_cast<int64_t>(KT_) * KT;                                         \
-        _Pragma("unroll") for (int j = 0; j < LA; ++j) ra[j] =                                      \
-            *reinterpret_cast<const v4i*>(Ag + j * rowstep + ko);       \
-        _Pragma("unroll") for (int j = 0; j < LB; ++j) rb[j] =                                      \
-            *reinterpret_cast<const v4i*>(Bg + j * rowstep + ko);       \
+    // stage KT_ (clamped to the last one: past the end the DMA refills a
+    // buffer that is never read again, keeping the vmcnt arithmetic uniform)
+#define OZ2_DMA(KT_, D_)                                                                                         \
+    {                                                                                                            \
+        char* st_ = sbase + ((KT_) % NST) * STAGE;                                                               \
+        const int64_t ko_ = static_cast<int64_t>(min((KT_), nkt - 1)) * 1024;                                    \
+        if ((D_) < LA)                                                                                           \
+            __builtin_amdgcn_global_load_lds((glb_ptr)(Ag + (D_) * bstride + ko_),                               \
+                                             (lds_ptr)(st_ + (wave * LA + (D_)) * 1024), 16, 0, 0);            \
+        else                                                                                                     \
+            __builtin_amdgcn_global_load_lds((glb_ptr)(Bg + ((D_) - LA) * bstride + ko_),                        \
+                                             (lds_ptr)(st_ + BM * KT + (wave * LB + (D_) - LA) * 1024), 16, 0, 0); \
     }
-#define OZ2_SSTORE(ST_)                                                                            \
-    {                                                                                              \
-        v4i* sa_ = smem + (ST_) * STAGE;                                                          \
-        v4i* sb_ = sa_ + BM * 8;                                                                  \
-        _Pragma("unroll") for (int j = 0; j < LA; ++j) {                                           \
-            const int row = lrow + 32 * j;                                                         \
-            sa_[row * 8 + swz(row, lchk)] = ra[j];                                                 \
-        }                                                                                          \
-        _Pragma("unroll") for (int j = 0; j < LB; ++j) {                                           \
-            const int row = lrow + 32 * j;                                                         \
-            sb_[row * 8 + swz(row, lchk)] = rb[j];                                                 \
-        }                                                                                          \
+#define OZ2_READ(NA, NB, KT_, R_)                                                                                \
+    {                                                                                                            \
+        const char* s_ = sbase + ((KT_) % NST) * STAGE;                                                          \
+        const int ks_ = (R_) / (FM + FN), w_ = (R_) % (FM + FN);                                                 \
+        if (w_ < FM) NA[ks_][w_] = *reinterpret_cast<const v4i*>(s_ + offa[ks_] + w_ * 2048);                   \
+        else NB[ks_][w_ - FM] = *reinterpret_cast<const v4i*>(s_ + offb[ks_] + (w_ - FM) * 2048);               \
+    }
+    // one stage: MFMAs on (FA, FB), reads of stage KTR into (NA, NB), DMAs of stage KTI,
+    // spread evenly in a fixed order
+#define OZ2_HALF(FA, FB, NA, NB, KTR, KTI)                                                                       \
+    {                                                                                                            \
+        _Pragma("unroll") for (int u = 0; u < NMF; ++u) {                                                        \
+            _Pragma("unroll") for (int r = 0; r < NR; ++r) if (r * NMF / NR == u) OZ2_READ(NA, NB, KTR, r)       \
+            __builtin_amdgcn_sched_barrier(0);                                                                   \
+            {                                                                                                    \
+                const int ks = u / (FM * FN), i = (u / FN) % FM, j = u % FN;                                     \
+                acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(FA[ks][i], FB[ks][j], acc[i][j], 0, 0, 0);     \
+            }                                                                                                    \
+            __builtin_amdgcn_sched_barrier(0);                                                                   \
+            _Pragma("unroll") for (int d = 0; d < ND; ++d) if (d * NMF / ND == u) OZ2_DMA(KTI, d)                \
+            __builtin_amdgcn_sched_barrier(0);                                                                   \
+        }                                                                                                        \
+    }
+    // top of a stage: its successor landed (own DMAs, then everyone's) and the
+    // fragments read during the previous stage are in registers
+#define OZ2_WAIT()                                                      \
+    {                                                                   \
+        __builtin_amdgcn_sched_barrier(0);                              \
+        __builtin_amdgcn_s_waitcnt(vm_wait(VM_AHEAD));                  \
+        __builtin_amdgcn_s_waitcnt(kLgkmWait0);                         \
+        __builtin_amdgcn_s_barrier();                                   \
+        __builtin_amdgcn_sched_barrier(0);                              \
     }
 
-    const int nkt = static_cast<int>(p.klen / KT);
-    OZ2_GLOAD(0)
-    OZ2_SSTORE(0)
-    __syncthreads();
-    const int fr = lane & 31, fc = lane >> 5;
-    for (int kt = 0; kt < nkt; ++kt) {
-        const int st = kt & 1;
-        if (kt + 1 < nkt) OZ2_GLOAD(kt + 1)
-        const v4i* sa = smem + st * STAGE;
-        const v4i* sb = sa + BM * 8;
+    // prologue: stages 0 .. NST-2 in flight, stage 0 landed and read
 #pragma unroll
-        for (int ks = 0; ks < KT / 32; ++ks) {
-            const int c = ks * 2 + fc;
-            v4i fa[FM], fb[FN];
+    for (int s0 = 0; s0 < NST - 1; ++s0)
 #pragma unroll
-            for (int i = 0; i < FM; ++i) {
-                const int row = wm * WTM + i * 32 + fr;
-                fa[i] = sa[row * 8 + swz(row, c)];
-            }
+        for (int d = 0; d < ND; ++d) OZ2_DMA(s0, d)
+    __builtin_amdgcn_s_waitcnt(vm_wait((NST - 2) * ND));
+    __builtin_amdgcn_s_barrier();
 #pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                const int row = wn * WTN + j * 32 + fr;
-                fb[j] = sb[row * 8 + swz(row, c)];
-            }
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-                for (int j = 0; j < FN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
-        }
-        if (kt + 1 < nkt) OZ2_SSTORE(st ^ 1)
-        __syncthreads();
-    }
-#undef OZ2_GLOAD
-#undef OZ2_SSTORE
+    for (int r = 0; r < NR; ++r) OZ2_READ(fa0, fb0, 0, r)
 
-    // epilogue: int32 -> residue mod m_t, 4 rows packed per 32-bit store
-    const int t = static_cast<int>(plane % static_cast<uint32_t>(p.s));
-    const TabSet* __restrict__ tb = p.tab;
-    uint8_t* Rb = p.R + static_cast<int64_t>(plane) * p.plane_r;
-    const int64_t col0 = static_cast<int64_t>(tn) * BN + wn * WTN + fr;
-    const int64_t row0 = static_cast<int64_t>(tm) * BM + wm * WTM + 4 * fc;
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                uint32_t w = 0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    w |= acc_residue(static_cast<uint32_t>(acc[i][j][4 * q + b]), tb, t) << (8 * b);
-                }
-                *reinterpret_cast<uint32_t*>(Rb + (col0 + j * 32) * p.ldr + row0 + i * 32 + 8 * q) = w;
-            }
+    for (int kt = 0; kt < nkt; kt += 2) {
+        OZ2_WAIT()
+        OZ2_HALF(fa0, fb0, fa1, fb1, kt + 1, kt + NST - 1)
+        OZ2_WAIT()
+        OZ2_HALF(fa1, fb1, fa0, fb0, kt + 2, kt + NST)  // the last pass reads a stage that is not used
+    }
+#undef OZ2_DMA
+#undef OZ2_READ
+#undef OZ2_HALF
+#undef OZ2_WAIT
+    __builtin_amdgcn_s_waitcnt(vm_wait(0));  // no DMA may still target LDS
+    __syncthreads();                          // every wave is done with the stage buffers
+    residue_epilogue<FM, FN>(p, acc, plane, static_cast<int64_t>(tm) * BM + wm * WTM,
+                             static_cast<int64_t>(tn) * BN + wn * WTN, sbase + wave * (WTM * WTN), lane);
 }
 
 }  // namespace
@@ -174,9 +253,9 @@ void igemm_residue(hipStream_t st, Tile tile, const GemmArgs& a) {
     const uint32_t nblk = static_cast<uint32_t>(a.tiles_m) * a.tiles_n * a.nplanes;
     if (nblk == 0 || a.klen <= 0) return;
     switch (tile) {
-        case Tile::T256: hipLaunchKernelGGL((igemm_residue_kernel<256, 256>), dim3(nblk), dim3(256), 0, st, a); break;
-        case Tile::T128: hipLaunchKernelGGL((igemm_residue_kernel<128, 128>), dim3(nblk), dim3(256), 0, st, a); break;
-        case Tile::T64: hipLaunchKernelGGL((igemm_residue_kernel<64, 64>), dim3(nblk), dim3(256), 0, st, a); break;
+        case Tile::T256: hipLaunchKernelGGL((igemm_residue_kernel<256, 256, 1>), dim3(nblk), dim3(256), 0, st, a); break;
+        case Tile::T128: hipLaunchKernelGGL((igemm_residue_kernel<128, 128, 2>), dim3(nblk), dim3(256), 0, st, a); break;
+        case Tile::T64: hipLaunchKernelGGL((igemm_residue_kernel<64, 64, 4>), dim3(nblk), dim3(256), 0, st, a); break;
     }
 }
 

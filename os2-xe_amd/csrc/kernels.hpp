@@ -1,8 +1,9 @@
 // kernels.hpp - host-side launch interface of the gfx950 kernels.
 //
 // Data layout in HBM (all owned by the engine's workspace):
-//   planes   int8  [group][t][row][kp]   row of op(A) (or column of op(B)) with k contiguous,
-//                                         rows padded with zeros to the GEMM tile, k to kp
+//   planes   int8  [group][t] then blocked: 1 KB per 16 rows x 64 k of op(A) (or op(B)^T),
+//                  blocks k-major within a 16-row band (plane_offset); rows padded with zeros to
+//                  the GEMM tile, k to kp
 //   residues uint8 [chunk][group][t][j][ldr]  GEMM results reduced mod m_t, i contiguous
 //   D, E     int32 exponents per row of op(A) / column of op(B)
 #pragma once
@@ -13,6 +14,16 @@
 
 namespace oz2 {
 namespace k {
+
+// Byte offset of element (row r, k p) inside one operand plane with kp columns:
+// 16-row bands of kp/64 consecutive 1 KB blocks; inside a block, 64 bytes per
+// row and the four 16-byte k-chunks of row r permuted by XOR with (r >> 2) & 3
+// (the GEMM's DMA copies blocks verbatim to LDS, where that permutation makes
+// the MFMA operand reads free of bank conflicts).
+__host__ __device__ inline int64_t plane_offset(int64_t r, int64_t p, int64_t kp) {
+    return (((r >> 4) * (kp >> 6) + (p >> 6)) << 10) + ((r & 15) << 6) +
+           (((((p >> 4) & 3) ^ ((r >> 2) & 3))) << 4) + (p & 15);
+}
 
 // ---- scaling -----------------------------------------------------------------
 // Exponents of the `len` rows of op(X) over k columns.  kcontig: row r of op(X)
@@ -43,8 +54,8 @@ void split(hipStream_t st, const SplitArgs& a);
 
 // ---- INT8 GEMM with residue epilogue ------------------------------------------
 struct GemmArgs {
-    const int8_t* A;      // planes of op(A): [nplanes][mpad][kp]
-    const int8_t* B;      // planes of op(B): [nplanes][npad][kp]
+    const int8_t* A;      // planes of op(A): nplanes x (mpad x kp, blocked by plane_offset)
+    const int8_t* B;      // planes of op(B): nplanes x (npad x kp, blocked by plane_offset)
     uint8_t* R;           // residues [nplanes][npad_c][ldr]
     int64_t kp;           // row stride of planes (bytes)
     int64_t plane_a, plane_b, plane_r;

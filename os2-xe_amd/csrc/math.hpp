@@ -147,12 +147,17 @@ OZ2_HD int32_t sym_value(const uint32_t (&a)[5], const TabSet* __restrict__ tb, 
     return static_cast<int32_t>(limb_residue<WIDE>(a, tb, t)) - static_cast<int32_t>(tb->h[t]);
 }
 
-// GEMM epilogue: int32 dot product -> residue in [0, m)
+// GEMM epilogue: int32 dot product x (as bits) -> x mod m in [0, m), m <= 255.
+// c31 = m ceil(2^31 / m) lies in [2^31, 2^31 + m), so x + c31 is x shifted into
+// [0, 2^32) without changing its residue for every x < 2^31 - 255; the GEMM's
+// accumulators satisfy |x| <= 130944 * 2^14 < 2^31 - 2^21.
+OZ2_HD uint32_t residue_m(uint32_t x, uint32_t m, uint32_t mu, uint32_t c16, uint32_t c31) {
+    return mod32(x + c31, m, mu, c16);
+}
 OZ2_HD uint32_t acc_residue(uint32_t x, const TabSet* __restrict__ tb, int t) {
     const uint32_t m = tb->m[t];
     if (m == 256) return x & 0xFFu;
-    uint32_t r = mod32(x ^ 0x80000000u, m, tb->mu[t], tb->c16[t]) + tb->k31[t];  // (x + 2^31) - 2^31 mod m
-    return umin(r, r - m);
+    return residue_m(x, m, tb->mu[t], tb->c16[t], tb->c31[t]);
 }
 
 OZ2_HD double ldexp_(double v, int e) {

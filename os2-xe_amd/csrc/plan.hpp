@@ -75,7 +75,7 @@ struct TabSet {
     uint32_t c16[kMaxModuli];               // 2^16 mod m
     uint32_t c21[kMaxModuli][4];            // 2^(21 j) mod m, j = 1..4
     uint32_t off[2][kMaxModuli];            // (m - 2^OFF mod m + h) mod m, OFF = 62 (narrow), 104 (wide)
-    uint32_t k31[kMaxModuli];               // (m - 2^31 mod m) mod m (GEMM residue epilogue)
+    uint32_t c31[kMaxModuli];               // m * ceil(2^31 / m) (GEMM residue epilogue offset; 0 for m = 256)
     // CRT (hpp:222-236)
     float frac[kMaxModuli];                 // fp32(y_t / m_t)
     uint32_t W16[kMaxModuli][2 * kMaxLimbs];  // 16-bit limbs of W_t

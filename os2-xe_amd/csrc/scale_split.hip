@@ -6,8 +6,10 @@
 // Inf/NaN) and hpp:251-303, 407-520 (x = RNE(v 2^e) taken exactly from the
 // IEEE bits, symmetric residues per modulus; complex: Re, +-Im, Re+-Im).
 //
-// Both operands are written in one plane layout, [t][row][kp] with k
-// contiguous, so the GEMM reads A and B tiles the same way.  Rows of op(X)
+// Both operands are written in one blocked plane layout (plane_offset: 1 KB
+// blocks of 16 rows x 64 k), so the GEMM copies A and B tiles the same way.
+// A thread's 16 (real) or 4 (complex) consecutive k stay inside one 16-byte
+// chunk of a block, so each plane store is one aligned vector store.  Rows of op(X)
 // that are strided in memory (op(A) = N, op(B) = T) are read column-by-column
 // with lanes along rows, so no separate transpose pass is needed.
 #include <algorithm>
@@ -142,7 +144,7 @@ __global__ void __launch_bounds__(256) split_real_kernel(SplitArgs a, int lg_tpr
         }
         to_limbs<WIDE>(bits, e, z, limbs[q]);
     }
-    int8_t* out = a.P + r * a.kp + p0;
+    int8_t* out = a.P + plane_offset(r, p0, a.kp);
     for (int t = 0; t < a.s; ++t) {
         uint32_t w[4];
 #pragma unroll
@@ -191,7 +193,7 @@ __global__ void __launch_bounds__(256) split_cplx_kernel(SplitArgs a, int lg_tpr
         to_limbs<WIDE>(br, e, z, lr[q]);
         to_limbs<WIDE>(bi, e, z, li[q]);
     }
-    int8_t* out = a.P + r * a.kp + p0;
+    int8_t* out = a.P + plane_offset(r, p0, a.kp);
     for (int t = 0; t < a.s; ++t) {
         const int32_t m = static_cast<int32_t>(tb->m[t]);
         const int32_t h = static_cast<int32_t>(tb->h[t]);

@@ -94,8 +94,9 @@ def test_split_complex_vs_oracle(s, tk, conj, rng):
 
 @pytest.mark.parametrize("s", [2, 7, 14, 16, 24])
 def test_acc_residue(s, rng):
-    x = rng.integers(-2 ** 31, 2 ** 31, 100000, dtype=np.int64)
-    x[:6] = [0, -1, 1, -2 ** 31, 2 ** 31 - 1, 130944 * 2 ** 14 - 1]
+    # documented domain x < 2^31 - 255 (GEMM accumulators: |x| <= 130944 * 2^14)
+    x = rng.integers(-2 ** 31, 2 ** 31 - 255, 100000, dtype=np.int64)
+    x[:8] = [0, -1, 1, -2 ** 31, 2 ** 31 - 256, 130944 * 2 ** 14, -130944 * 2 ** 14, 130944 * 127 * 127]
     out = np.zeros((s, x.size), dtype=np.uint8)
     xi = x.astype(np.int32)
     _emu.emu_acc_residue(P(xi), x.size, s, P(out))

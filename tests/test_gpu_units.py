@@ -30,6 +30,15 @@ def h2f(h):
     return struct.unpack("<d", struct.pack("<Q", h))[0]
 
 
+def plane_index(rows_pad, kp):
+    """flat offset of (row, k) in one blocked plane: 16-row bands of 1 KB blocks (64 k each),
+    64 bytes per row, 16-byte k-chunks XOR-permuted by (row >> 2) & 3 (oz2_c.h, oz2_unit_split)"""
+    r = np.arange(rows_pad)[:, None]
+    p = np.arange(kp)[None, :]
+    return ((((r >> 4) * (kp >> 6) + (p >> 6)) << 10) + ((r & 15) << 6) + ((((p >> 4) & 3) ^ ((r >> 2) & 3)) << 4)
+            + (p & 15))
+
+
 def run_split(values, ex, s, tk, kcontig, cplx=False, conj=False):
     """values: rows x k (complex if cplx); returns planes [g][t][rows][k]"""
     rows, k = values.shape
@@ -48,7 +57,10 @@ def run_split(values, ex, s, tk, kcontig, cplx=False, conj=False):
     oz2.unit_split(to_dev(X.reshape(-1)), ldx, rows, k, kcontig, to_dev(np.asarray(ex, dtype=np.int32)), s, tk, P,
                    rows_pad, kp, cplx=cplx, conj=conj)
     torch.cuda.synchronize()
-    P = P.cpu().numpy().reshape(g, s, rows_pad, kp)
+    flat = P.cpu().numpy().reshape(g, s, rows_pad * kp)
+    idx = plane_index(rows_pad, kp)
+    assert np.array_equal(np.sort(idx.reshape(-1)), np.arange(rows_pad * kp)), "layout must be a permutation"
+    P = flat[:, :, idx]
     assert not P[:, :, rows:, :].any() and not P[:, :, :, k:].any(), "padding must be zero"
     return P[:, :, :rows, :k]
 
