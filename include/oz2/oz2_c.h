@@ -82,6 +82,72 @@ int oz2_profile_end(oz2_handle h, oz2_stage_times_t* sum, int64_t* calls);
 /* the moduli of a set (coprime_tables.hpp); writes num_moduli values, returns OZ2_INVALID_ARGUMENT if out of range */
 int oz2_moduli(int num_moduli, uint16_t* out);
 
+/* ---- Batches of DGEMMs (no reference counterpart: the reference runs one GEMM per call) ----
+ * A plan holds a list of DGEMMs, each with the full oz2_dgemm semantics (op, alpha, beta, leading
+ * dimensions, num_moduli).  num_moduli > 0 fixes a problem's moduli; num_moduli <= 0 lets the plan
+ * choose: the learned selector when OZ2_BATCH_SELECT is set (falling back to moduli_for_bits when it
+ * is not confident), else moduli_for_bits(k, bits).  The plan owns one workspace for all problems and
+ * a grouped launch sequence (scale, split, one persistent INT8 GEMM per macro-tile shape, CRT: about
+ * ten launches for any count); with OZ2_BATCH_GRAPH that sequence is captured in a hipGraph when the
+ * plan is created and oz2_batch_run replays it.  Buffers are bound at creation: running the plan
+ * again recomputes from their current contents.  Each problem's C is bit-identical to oz2_dgemm with
+ * the same num_moduli.  tiles: NULL (per-problem heuristic) or count entries of 256, 128, 64 or 0
+ * (heuristic).  oz2_batch_run enqueues on the handle's current stream; oz2_batch_destroy waits for
+ * the device first.  Real DGEMMs only. */
+typedef struct {
+    int ta, tb;
+    int64_t m, n, k;
+    double alpha;
+    const double* A;
+    int64_t lda;
+    const double* B;
+    int64_t ldb;
+    double beta;
+    double* C;
+    int64_t ldc;
+    int num_moduli;
+} oz2_gemm_desc_t;
+typedef struct oz2_batch_s* oz2_batch;
+enum { OZ2_BATCH_GRAPH = 1, OZ2_BATCH_SELECT = 2 };
+int oz2_batch_create(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, const int* tiles, int flags,
+                     oz2_batch* out);
+int oz2_batch_run(oz2_batch b);
+/* chosen moduli / macro tiles per problem (count ints each) */
+int oz2_batch_moduli(oz2_batch b, int* out);
+int oz2_batch_tiles(oz2_batch b, int* out);
+/* workspace bytes, kernel launches per run, 1 if replayed from a graph */
+int oz2_batch_info(oz2_batch b, size_t* workspace_bytes, int* launches, int* graph);
+int oz2_batch_destroy(oz2_batch b);
+
+/* ---- Learned moduli selector (no reference counterpart; moduli_for_bits, hpp:693-700, is its fallback) ----
+ * Features: 48 floats per problem from sampled exponent gaps of op(A) rows / op(B) columns plus log2 k
+ * and the bit target ("b bits": every entry within 2^(53-b) ulp of the exact product).  The network
+ * (48 -> 64 -> 64 -> 23, FP32, OZ2_SEL_PARAMS floats) gives for s = 2..24 the probability that s moduli
+ * reach the target; the choice is the smallest s whose probability reaches the threshold, else
+ * moduli_for_bits(k, bits).  num_moduli in the descriptors is ignored here.  These calls synchronize
+ * the handle's stream; out arrays are host memory. */
+enum { OZ2_SEL_FEATURES = 48, OZ2_SEL_OUTPUTS = 23, OZ2_SEL_PARAMS = 8791 };
+int oz2_selector_features(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, float* features);
+/* probs: count x 23 (zeros when no weights are loaded); s_out: count chosen s; learned: 1 where the
+ * network chose, 0 where the fallback did (either may be NULL) */
+int oz2_selector_choose(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, float* probs, int* s_out,
+                        int* learned);
+/* n = OZ2_SEL_PARAMS host floats (n = 0: unload), threshold in (0, 1) */
+int oz2_selector_set_weights(oz2_handle h, const float* weights, size_t n, float threshold);
+/* copies the loaded weights (returns the count, 0 if none) and the threshold */
+int oz2_selector_get_weights(oz2_handle h, float* weights, size_t n, float* threshold, size_t* count);
+/* Training kernels on device arrays, enqueued on the handle's stream (layout: W1[64][48] b1 W2[64][64] b2
+ * W3[23][64] b3, row-major).  forward: X nb x 48 -> H1, H2 nb x 64, Z logits and P = sigmoid(Z) nb x 23
+ * (any output may be NULL).  backward: masked binary cross-entropy summed and scaled by inv_norm;
+ * D1, D2 nb x 64 and D3 nb x 23 scratch; G the gradient (OZ2_SEL_PARAMS); loss nb x 23 terms or NULL.
+ * adam: one Adam step (step >= 1) with decoupled-free L2 term wd*W added to the gradient. */
+int oz2_mlp_forward(oz2_handle h, const float* X, int nb, const float* W, float* H1, float* H2, float* Z, float* P);
+int oz2_mlp_backward(oz2_handle h, const float* X, int nb, const float* W, const float* H1, const float* H2,
+                     const float* Z, const float* Y, const float* M, float inv_norm, float* D1, float* D2, float* D3,
+                     float* G, float* loss);
+int oz2_mlp_adam(oz2_handle h, float* W, const float* G, float* M, float* V, int step, float lr, float beta1,
+                 float beta2, float eps, float wd);
+
 /* Kernel-level entry points, the counterparts of the reference's detail::launch_split and
  * detail::launch_crt as its unit tests call them (tests/test_ozaki2.cpp:215-221, 306-313).
  * oz2_unit_split: rows of op(X) (row r, column p at X[r*ldx + p] if kcontig, else X[r + p*ldx];

@@ -1,0 +1,275 @@
+// batch.cpp - grouped launch sequence for many DGEMMs (see batch.hpp).
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+
+namespace oz2 {
+
+namespace {
+inline int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("oz2: ") + what + ": " + hipGetErrorString(e));
+}
+k::Tile tile_of_rows(int t) { return t == 256 ? k::Tile::T256 : t == 128 ? k::Tile::T128 : k::Tile::T64; }
+}  // namespace
+
+BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector<int>& s,
+                     const std::vector<int>& tiles, bool use_graph)
+    : eng_(eng), d_(std::move(descs)) {
+    const size_t np = d_.size();
+    if (s.size() != np) throw std::invalid_argument("oz2: batch needs one number of moduli per problem");
+    if (!tiles.empty() && tiles.size() != np) throw std::invalid_argument("oz2: batch needs one tile per problem");
+    s_ = s;
+    tile_rows_.assign(np, 0);
+    // validation (hpp:745-748, per problem)
+    for (size_t i = 0; i < np; ++i) {
+        const GemmDesc& g = d_[i];
+        const std::string at = " (batch entry " + std::to_string(i) + ")";
+        if (g.m < 0 || g.n < 0 || g.k < 0) throw std::invalid_argument("oz2: negative dimension" + at);
+        if (g.lda < std::max<int64_t>(1, g.ta == Op::N ? g.m : g.k) ||
+            g.ldb < std::max<int64_t>(1, g.tb == Op::N ? g.k : g.n) || g.ldc < std::max<int64_t>(1, g.m))
+            throw std::invalid_argument("oz2: bad leading dimension" + at);
+        if (s_[i] < kMinModuli || s_[i] > kMaxModuli)
+            throw std::invalid_argument("oz2: num_moduli must be in [2, 24]" + at);
+    }
+
+    // per-problem layout inside one workspace: D, E, partial maxima, planes, residues
+    struct P {
+        Plan plan;
+        int64_t T, kp, kch, mpad, npad, planeA, planeB, planeR;
+        size_t oD, oE, oPA, oPB, oA, oB, oR;
+    };
+    std::vector<P> pp(np);
+    std::vector<size_t> work;  // indices of problems that need the full path
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    for (size_t i = 0; i < np; ++i) {
+        const GemmDesc& g = d_[i];
+        if (g.m == 0 || g.n == 0) continue;
+        if (g.k == 0 || g.alpha == 0.0) {
+            special_.push_back(i);
+            continue;
+        }
+        P& q = pp[i];
+        q.plan = make_plan(s_[i], g.k);  // throws invalid_argument when s is too small for k (hpp:186)
+        const int planes = s_[i];
+        const k::Tile t = tiles.empty() || tiles[i] <= 0 ? eng_.pick_tile(g.m, g.n, planes) : tile_of_rows(tiles[i]);
+        q.T = k::tile_rows(t);
+        tile_rows_[i] = static_cast<int>(q.T);
+        q.kp = round_up(g.k, kKStep);
+        q.kch = (q.kp + kMaxKChunk - 1) / kMaxKChunk;
+        q.mpad = round_up(g.m, q.T);
+        q.npad = round_up(g.n, q.T);
+        q.planeA = q.mpad * q.kp;
+        q.planeB = q.npad * q.kp;
+        q.planeR = q.npad * q.mpad;
+        q.oD = carve(static_cast<size_t>(g.m) * 4);
+        q.oE = carve(static_cast<size_t>(g.n) * 4);
+        q.oPA = carve(k::scale_partial_elems(g.m, g.k, g.ta != Op::N) * 8);
+        q.oPB = carve(k::scale_partial_elems(g.n, g.k, g.tb == Op::N) * 8);
+        q.oA = carve(static_cast<size_t>(planes) * q.planeA);
+        q.oB = carve(static_cast<size_t>(planes) * q.planeB);
+        q.oR = carve(static_cast<size_t>(q.kch) * planes * q.planeR);
+        work.push_back(i);
+    }
+    const size_t data_bytes = off;
+
+    // grouped launches; GEMM work listed largest first so the persistent grid's tail is short
+    std::vector<k::ScaleJob> sj;
+    std::vector<k::SplitArgs> pj;
+    std::vector<k::GemmJob> gj;
+    std::vector<k::CrtArgs> cj;
+    std::vector<size_t> by_cost = work;
+    std::stable_sort(by_cost.begin(), by_cost.end(), [&](size_t a, size_t b) {
+        auto cost = [&](size_t i) { return static_cast<double>(pp[i].mpad) * pp[i].npad * pp[i].kp * s_[i]; };
+        return cost(a) > cost(b);
+    });
+    const TabSet* tabs = eng_.device_tables();
+    // the jobs hold device pointers into the workspace, whose size depends on the
+    // job tables: build once against a null base to size them, then for real
+    auto build = [&](std::byte* base) {
+        sj.clear();
+        pj.clear();
+        gj.clear();
+        cj.clear();
+        for (size_t i : work) {
+            const GemmDesc& g = d_[i];
+            const P& q = pp[i];
+            const int tk = q.plan.tk;
+            int32_t* D = reinterpret_cast<int32_t*>(base + q.oD);
+            int32_t* E = reinterpret_cast<int32_t*>(base + q.oE);
+            sj.push_back({g.A, g.lda, g.m, g.k, g.ta != Op::N, false, tk, reinterpret_cast<uint64_t*>(base + q.oPA), D});
+            sj.push_back({g.B, g.ldb, g.n, g.k, g.tb == Op::N, false, tk, reinterpret_cast<uint64_t*>(base + q.oPB), E});
+            for (int which = 0; which < 2; ++which) {
+                k::SplitArgs a{};
+                a.kcontig = which == 0 ? g.ta != Op::N : g.tb == Op::N;
+                a.X = which == 0 ? g.A : g.B;
+                a.ldx = which == 0 ? g.lda : g.ldb;
+                a.rows = which == 0 ? g.m : g.n;
+                a.rows_pad = which == 0 ? q.mpad : q.npad;
+                a.k = g.k;
+                a.kp = q.kp;
+                a.ex = which == 0 ? D : E;
+                a.P = reinterpret_cast<int8_t*>(base + (which == 0 ? q.oA : q.oB));
+                a.plane = which == 0 ? q.planeA : q.planeB;
+                a.group = static_cast<int64_t>(s_[i]) * a.plane;
+                a.s = s_[i];
+                a.cplx = false;
+                a.conj = false;
+                a.wide = tk > kNarrowScaleBits;
+                a.tab = tabs + s_[i];
+                pj.push_back(a);
+            }
+            k::CrtArgs c{};
+            c.R = reinterpret_cast<const uint8_t*>(base + q.oR);
+            c.plane_r = q.planeR;
+            c.chunk_r = static_cast<int64_t>(s_[i]) * q.planeR;
+            c.kch = static_cast<int>(q.kch);
+            c.ldr = q.mpad;
+            c.mb = g.m;
+            c.nb = g.n;
+            c.D = D;
+            c.E = E;
+            c.C = g.C;
+            c.ldc = g.ldc;
+            c.ar = g.alpha;
+            c.ai = 0.0;
+            c.br = g.beta;
+            c.bi = 0.0;
+            c.cplx = false;
+            c.s = s_[i];
+            c.L16 = 2 * q.plan.NL;
+            c.tab = tabs + s_[i];
+            cj.push_back(c);
+        }
+        for (size_t i : by_cost) {
+            const P& q = pp[i];
+            for (int64_t c = 0; c < q.kch; ++c) {
+                k::GemmArgs a{};
+                a.A = reinterpret_cast<const int8_t*>(base + q.oA);
+                a.B = reinterpret_cast<const int8_t*>(base + q.oB);
+                a.kp = q.kp;
+                a.plane_a = q.planeA;
+                a.plane_b = q.planeB;
+                a.plane_r = q.planeR;
+                a.ldr = q.mpad;
+                a.k0 = c * kMaxKChunk;
+                a.klen = std::min(kMaxKChunk, q.kp - a.k0);
+                a.R = reinterpret_cast<uint8_t*>(base + q.oR) + c * s_[i] * q.planeR;
+                a.tiles_m = static_cast<int>(q.mpad / q.T);
+                a.tiles_n = static_cast<int>(q.npad / q.T);
+                a.nplanes = s_[i];
+                a.s = s_[i];
+                a.tab = tabs + s_[i];
+                gj.push_back({tile_of_rows(static_cast<int>(q.T)), a});
+            }
+        }
+        scale_.clear();
+        split_.clear();
+        gemm_.clear();
+        crt_.clear();
+        k::prep_scale(scale_, sj);
+        k::prep_split(split_, pj);
+        k::prep_gemm(gemm_, gj);
+        k::prep_crt(crt_, cj);
+    };
+    build(nullptr);
+    size_t tables = 0;
+    for (auto* v : {&scale_, &split_, &gemm_, &crt_})
+        for (const k::GroupedLaunch& g : *v)
+            tables += align_up(g.items.size(), 256) + align_up(g.start.size() * 4, 256);
+    const size_t ncounters = gemm_.size();
+    ws_size_ = data_bytes + tables + align_up(std::max<size_t>(1, ncounters) * 4, 256) + 4096;
+    if (ws_size_ > eng_.budget())
+        throw std::runtime_error("oz2: not enough device memory for the batch workspace (" +
+                                 std::to_string(ws_size_ >> 20) + " MiB)");
+    void* p = nullptr;
+    if (hipMalloc(&p, ws_size_) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        throw std::runtime_error("oz2: batch workspace allocation failed");
+    }
+    ws_ = static_cast<std::byte*>(p);
+    build(ws_);
+    // upload the tables (once; the captured graph only reads them)
+    size_t toff = data_bytes;
+    std::vector<std::byte> host(tables);
+    for (auto* v : {&scale_, &split_, &gemm_, &crt_})
+        for (k::GroupedLaunch& g : *v) {
+            std::memcpy(host.data() + (toff - data_bytes), g.items.data(), g.items.size());
+            g.d_items = ws_ + toff;
+            toff += align_up(g.items.size(), 256);
+            std::memcpy(host.data() + (toff - data_bytes), g.start.data(), g.start.size() * 4);
+            g.d_start = reinterpret_cast<const uint32_t*>(ws_ + toff);
+            toff += align_up(g.start.size() * 4, 256);
+        }
+    counters_ = reinterpret_cast<uint32_t*>(ws_ + toff);
+    if (tables) hip_check(hipMemcpy(ws_ + data_bytes, host.data(), tables, hipMemcpyHostToDevice), "batch table upload");
+
+    if (use_graph) {
+        hipStream_t cs = nullptr;
+        hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream create");
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            enqueue(cs);
+            e = hipStreamEndCapture(cs, &graph_);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(cs);
+        if (e != hipSuccess) {
+            if (graph_) (void)hipGraphDestroy(graph_);
+            graph_ = nullptr;
+            exec_ = nullptr;
+            (void)hipFree(ws_);
+            ws_ = nullptr;
+            hip_check(e, "batch graph capture");
+        }
+    }
+}
+
+BatchPlan::~BatchPlan() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (ws_) {
+        (void)hipDeviceSynchronize();  // the plan may have been run on any stream
+        (void)hipFree(ws_);
+    }
+}
+
+int BatchPlan::launches() const {
+    int n = static_cast<int>(special_.size());
+    for (auto* v : {&scale_, &split_, &crt_}) n += static_cast<int>(v->size());
+    return n + 2 * static_cast<int>(gemm_.size());  // each GEMM launch follows a counter reset
+}
+
+void BatchPlan::enqueue(hipStream_t st) {
+    for (size_t i : special_) {
+        const GemmDesc& g = d_[i];
+        k::scale_c(st, g.C, g.ldc, g.m, g.n, g.beta, 0.0, false);
+    }
+    for (const k::GroupedLaunch& g : scale_) k::launch_scale_split_grouped(st, g);
+    for (const k::GroupedLaunch& g : split_) k::launch_scale_split_grouped(st, g);
+    for (size_t i = 0; i < gemm_.size(); ++i) k::launch_gemm_grouped(st, gemm_[i], counters_ + i, eng_.compute_units());
+    for (const k::GroupedLaunch& g : crt_) k::launch_crt_grouped(st, g);
+}
+
+void BatchPlan::run(hipStream_t st) {
+    if (exec_) hip_check(hipGraphLaunch(exec_, st), "batch graph launch");
+    else enqueue(st);
+    hip_check(hipGetLastError(), "batch launch");
+}
+
+void BatchPlan::run_gemm_only(hipStream_t st) {
+    for (size_t i = 0; i < gemm_.size(); ++i) k::launch_gemm_grouped(st, gemm_[i], counters_ + i, eng_.compute_units());
+    hip_check(hipGetLastError(), "batch gemm launch");
+}
+
+}  // namespace oz2

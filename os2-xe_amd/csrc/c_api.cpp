@@ -4,6 +4,7 @@
 #include <algorithm>
 
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -11,11 +12,20 @@
 #include <string>
 
 #include "../../include/oz2/oz2_c.h"
+#include "batch.hpp"
+#include "selector.hpp"
 #include "engine.hpp"
 
 struct oz2_engine {
     oz2::Engine eng;
+    oz2::Selector sel;
     explicit oz2_engine(hipStream_t s) : eng(s) {}
+};
+
+struct oz2_batch_s {
+    oz2_engine* h = nullptr;
+    std::unique_ptr<oz2::BatchPlan> plan;
+    int count = 0;
 };
 
 namespace {
@@ -56,6 +66,37 @@ oz2::Op to_op(int t) {
 
 void need(oz2_handle h) {
     if (!h) throw std::invalid_argument("oz2: null handle");
+}
+
+oz2::SelProblem sel_problem(const oz2::GemmDesc& g, int bits) {
+    return oz2::SelProblem{g.ta == oz2::Op::N ? 0 : 1, g.tb == oz2::Op::N ? 0 : 1, g.m, g.n, g.k, g.A, g.lda, g.B,
+                           g.ldb, bits};
+}
+
+std::vector<oz2::GemmDesc> descs_of(int count, const oz2_gemm_desc_t* descs) {
+    if (count < 0 || (count > 0 && !descs)) throw std::invalid_argument("oz2: bad problem list");
+    std::vector<oz2::GemmDesc> d(count);
+    for (int i = 0; i < count; ++i) {
+        const oz2_gemm_desc_t& c = descs[i];
+        if (c.m < 0 || c.n < 0 || c.k < 0) throw std::invalid_argument("oz2: negative dimension");
+        d[i] = oz2::GemmDesc{to_op(c.ta), to_op(c.tb), c.m, c.n, c.k, c.alpha, c.A, c.lda,
+                             c.B, c.ldb, c.beta, c.C, c.ldc, c.num_moduli};
+    }
+    return d;
+}
+
+// learned choice of s for the problems with num_moduli <= 0 (moduli_for_bits where the model abstains)
+void select_moduli(oz2_handle h, const std::vector<oz2::GemmDesc>& d, int bits, std::vector<int>& s) {
+    std::vector<oz2::SelProblem> probs;
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < d.size(); ++i)
+        if (d[i].s <= 0 && d[i].m > 0 && d[i].n > 0 && d[i].k > 0) {
+            probs.push_back(sel_problem(d[i], bits));
+            idx.push_back(i);
+        }
+    if (probs.empty()) return;
+    const std::vector<int> c = h->sel.choose(h->eng.stream(), probs);
+    for (size_t j = 0; j < idx.size(); ++j) s[idx[j]] = c[j];
 }
 
 void wait_deps(hipStream_t st, void* const* deps, int ndeps) {
@@ -149,16 +190,7 @@ int oz2_precision_bits_complex(int64_t k, int num_moduli, int* err) {
     return oz2_precision_bits(2 * k, num_moduli, err);
 }
 
-int oz2_moduli_for_bits(int64_t k, int bits) {
-    // smallest s whose plan keeps >= bits; 24 if none does (hpp:693-700)
-    for (int s = oz2::kMinModuli; s <= oz2::kMaxModuli; ++s) {
-        try {
-            if (oz2::make_plan(s, k).tk >= bits) return s;
-        } catch (const std::invalid_argument&) {
-        }
-    }
-    return oz2::kMaxModuli;
-}
+int oz2_moduli_for_bits(int64_t k, int bits) { return oz2::moduli_for_bits(k, bits); }
 
 int oz2_set_workspace_limit(oz2_handle h, size_t bytes) {
     return guard([&] { need(h); h->eng.set_workspace_limit(bytes); });
@@ -252,6 +284,149 @@ int oz2_moduli(int num_moduli, uint16_t* out) {
         if (!ms) throw std::invalid_argument("oz2: num_moduli must be in [2, 24]");
         if (!out) throw std::invalid_argument("oz2: null output pointer");
         for (int t = 0; t < num_moduli; ++t) out[t] = ms[t];
+    });
+}
+
+int oz2_batch_create(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, const int* tiles, int flags,
+                     oz2_batch* out) {
+    return guard([&] {
+        need(h);
+        if (!out) throw std::invalid_argument("oz2: null output");
+        if (count < 0 || (count > 0 && !descs)) throw std::invalid_argument("oz2: bad batch");
+        std::vector<oz2::GemmDesc> d(count);
+        std::vector<int> s(count), t;
+        for (int i = 0; i < count; ++i) {
+            const oz2_gemm_desc_t& c = descs[i];
+            d[i] = oz2::GemmDesc{to_op(c.ta), to_op(c.tb), c.m, c.n, c.k, c.alpha, c.A, c.lda,
+                                 c.B, c.ldb, c.beta, c.C, c.ldc, c.num_moduli};
+            s[i] = c.num_moduli > 0 ? c.num_moduli : oz2::moduli_for_bits(std::max<int64_t>(c.k, 1), bits);
+        }
+        // validation of shapes happens in the plan; the selector only reads problems that pass it
+        for (const auto& g : d)
+            if (g.m < 0 || g.n < 0 || g.k < 0) throw std::invalid_argument("oz2: negative dimension");
+        if ((flags & OZ2_BATCH_SELECT) != 0) select_moduli(h, d, bits, s);
+        if (tiles) t.assign(tiles, tiles + count);
+        auto b = std::make_unique<oz2_batch_s>();
+        b->h = h;
+        b->count = count;
+        b->plan = std::make_unique<oz2::BatchPlan>(h->eng, std::move(d), s, t, (flags & OZ2_BATCH_GRAPH) != 0);
+        *out = b.release();
+    });
+}
+
+int oz2_batch_run(oz2_batch b) {
+    return guard([&] {
+        if (!b) throw std::invalid_argument("oz2: null batch");
+        b->plan->run(b->h->eng.stream());
+    });
+}
+
+int oz2_batch_moduli(oz2_batch b, int* out) {
+    return guard([&] {
+        if (!b || !out) throw std::invalid_argument("oz2: null argument");
+        std::copy(b->plan->moduli().begin(), b->plan->moduli().end(), out);
+    });
+}
+
+int oz2_batch_tiles(oz2_batch b, int* out) {
+    return guard([&] {
+        if (!b || !out) throw std::invalid_argument("oz2: null argument");
+        std::copy(b->plan->tiles().begin(), b->plan->tiles().end(), out);
+    });
+}
+
+int oz2_batch_info(oz2_batch b, size_t* workspace_bytes, int* launches, int* graph) {
+    return guard([&] {
+        if (!b) throw std::invalid_argument("oz2: null batch");
+        if (workspace_bytes) *workspace_bytes = b->plan->workspace_bytes();
+        if (launches) *launches = b->plan->launches();
+        if (graph) *graph = b->plan->uses_graph() ? 1 : 0;
+    });
+}
+
+int oz2_batch_destroy(oz2_batch b) {
+    return guard([&] { delete b; });
+}
+
+int oz2_selector_features(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, float* features) {
+    return guard([&] {
+        need(h);
+        if (!features && count > 0) throw std::invalid_argument("oz2: null output");
+        const std::vector<oz2::GemmDesc> d = descs_of(count, descs);
+        std::vector<oz2::SelProblem> probs;
+        for (const auto& g : d) probs.push_back(sel_problem(g, bits));
+        std::vector<float> f;
+        h->sel.evaluate(h->eng.stream(), probs, &f, nullptr);
+        std::copy(f.begin(), f.end(), features);
+    });
+}
+
+int oz2_selector_choose(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, float* probs, int* s_out,
+                        int* learned) {
+    return guard([&] {
+        need(h);
+        const std::vector<oz2::GemmDesc> d = descs_of(count, descs);
+        std::vector<oz2::SelProblem> pr;
+        for (const auto& g : d) pr.push_back(sel_problem(g, bits));
+        if (probs) {
+            std::vector<float> p;
+            h->sel.evaluate(h->eng.stream(), pr, nullptr, &p);
+            std::copy(p.begin(), p.end(), probs);
+        }
+        std::vector<bool> l;
+        const std::vector<int> s = h->sel.choose(h->eng.stream(), pr, &l);
+        if (s_out) std::copy(s.begin(), s.end(), s_out);
+        if (learned)
+            for (size_t i = 0; i < l.size(); ++i) learned[i] = l[i] ? 1 : 0;
+    });
+}
+
+int oz2_selector_set_weights(oz2_handle h, const float* weights, size_t n, float threshold) {
+    return guard([&] {
+        need(h);
+        if (n && !weights) throw std::invalid_argument("oz2: null weights");
+        h->sel.set(weights, n, threshold);
+    });
+}
+
+int oz2_selector_get_weights(oz2_handle h, float* weights, size_t n, float* threshold, size_t* count) {
+    return guard([&] {
+        need(h);
+        const auto& w = h->sel.weights();
+        if (weights && n) std::copy(w.begin(), w.begin() + std::min(n, w.size()), weights);
+        if (threshold) *threshold = h->sel.threshold();
+        if (count) *count = w.size();
+    });
+}
+
+int oz2_mlp_forward(oz2_handle h, const float* X, int nb, const float* W, float* H1, float* H2, float* Z, float* P) {
+    return guard([&] {
+        need(h);
+        if (nb < 0 || (nb > 0 && (!X || !W))) throw std::invalid_argument("oz2: bad mlp arguments");
+        oz2::k::mlp_forward(h->eng.stream(), X, nb, W, H1, H2, Z, P);
+        if (hipGetLastError() != hipSuccess) throw std::runtime_error("oz2: mlp launch failed");
+    });
+}
+
+int oz2_mlp_backward(oz2_handle h, const float* X, int nb, const float* W, const float* H1, const float* H2,
+                     const float* Z, const float* Y, const float* M, float inv_norm, float* D1, float* D2, float* D3,
+                     float* G, float* loss) {
+    return guard([&] {
+        need(h);
+        if (nb < 0 || (nb > 0 && (!X || !W || !H1 || !H2 || !Z || !Y || !D1 || !D2 || !D3 || !G)))
+            throw std::invalid_argument("oz2: bad mlp arguments");
+        oz2::k::mlp_backward(h->eng.stream(), X, nb, W, H1, H2, Z, Y, M, inv_norm, D1, D2, D3, G, loss);
+        if (hipGetLastError() != hipSuccess) throw std::runtime_error("oz2: mlp launch failed");
+    });
+}
+
+int oz2_mlp_adam(oz2_handle h, float* W, const float* G, float* M, float* V, int step, float lr, float beta1,
+                 float beta2, float eps, float wd) {
+    return guard([&] {
+        need(h);
+        if (!W || !G || !M || !V || step < 1) throw std::invalid_argument("oz2: bad adam arguments");
+        oz2::k::mlp_adam(h->eng.stream(), W, G, M, V, step, lr, beta1, beta2, eps, wd);
+        if (hipGetLastError() != hipSuccess) throw std::runtime_error("oz2: mlp launch failed");
     });
 }
 

@@ -11,6 +11,7 @@
 // C = alpha*v (beta == 0, C not read) or fma(alpha, v, beta*C); complex:
 // Re = P0 - P1, Im = P2 - P0 - P1 in residue space (3M form), then
 // (ar*vr - ai*vi, ar*vi + ai*vr) + beta*C, each product pair fused as one fma.
+#include "grouped.hpp"
 #include "kernels.hpp"
 #include "math.hpp"
 
@@ -29,10 +30,10 @@ __device__ __forceinline__ uint32_t chunk_sum(uint32_t s, uint32_t m) {
 
 // 4 consecutive rows per thread; residues are bytes packed 4 per 32-bit load.
 template <int NL, bool CPLX>
-__global__ void __launch_bounds__(256) crt_kernel(CrtArgs a) {
+__device__ __forceinline__ void crt_body(const CrtArgs& a, uint32_t bx) {
     const TabSet* __restrict__ tb = a.tab;
     const int64_t mq = (a.mb + 3) >> 2;
-    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t idx = static_cast<int64_t>(bx) * 256 + threadIdx.x;
     const int64_t j = idx / mq;
     const int64_t i0 = (idx - j * mq) * 4;
     if (j >= a.nb) return;
@@ -112,6 +113,23 @@ __global__ void __launch_bounds__(256) crt_kernel(CrtArgs a) {
     }
 }
 
+template <int NL, bool CPLX>
+__global__ void __launch_bounds__(256) crt_kernel(CrtArgs a) {
+    crt_body<NL, CPLX>(a, blockIdx.x);
+}
+
+template <int NL, bool CPLX>
+__global__ void __launch_bounds__(256) crt_grouped_kernel(const CrtArgs* __restrict__ items,
+                                                          const uint32_t* __restrict__ start, uint32_t n) {
+    const uint32_t p = owner_of(start, n, blockIdx.x);
+    crt_body<NL, CPLX>(items[p], blockIdx.x - start[p]);
+}
+
+uint32_t crt_blocks(const CrtArgs& a) {
+    const int64_t threads = ((a.mb + 3) / 4) * a.nb;
+    return static_cast<uint32_t>((threads + 255) / 256);
+}
+
 __global__ void scale_c_kernel(double* C, int64_t ldc, int64_t m, int64_t n, double br, double bi, int cplx) {
     const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= m * n) return;
@@ -134,8 +152,7 @@ __global__ void scale_c_kernel(double* C, int64_t ldc, int64_t m, int64_t n, dou
 
 template <int NL>
 void launch_crt_nl(hipStream_t st, const CrtArgs& a) {
-    const int64_t threads = ((a.mb + 3) / 4) * a.nb;
-    const unsigned blocks = static_cast<unsigned>((threads + 255) / 256);
+    const unsigned blocks = crt_blocks(a);
     if (a.cplx) hipLaunchKernelGGL((crt_kernel<NL, true>), dim3(blocks), dim3(256), 0, st, a);
     else hipLaunchKernelGGL((crt_kernel<NL, false>), dim3(blocks), dim3(256), 0, st, a);
 }
@@ -150,6 +167,36 @@ void crt(hipStream_t st, const CrtArgs& a) {
         case 4: launch_crt_nl<4>(st, a); break;
         case 5: launch_crt_nl<5>(st, a); break;
         case 6: launch_crt_nl<6>(st, a); break;
+        default: break;
+    }
+}
+
+// grouped CRT: one launch per (limb count, real/complex)
+void prep_crt(std::vector<GroupedLaunch>& out, const std::vector<CrtArgs>& jobs) {
+    GroupedLaunch by[14];
+    for (const CrtArgs& a : jobs) {
+        if (a.mb <= 0 || a.nb <= 0) continue;
+        const int v = (a.L16 / 2) * 2 + (a.cplx ? 1 : 0);
+        by[v].kind = v;
+        by[v].add(a, crt_blocks(a));
+    }
+    for (GroupedLaunch& g : by)
+        if (g.n) out.push_back(std::move(g));
+}
+
+void launch_crt_grouped(hipStream_t st, const GroupedLaunch& g) {
+    if (g.n == 0) return;
+    const CrtArgs* items = static_cast<const CrtArgs*>(g.d_items);
+    switch (g.kind) {
+#define OZ2_NL(NL)                                                                                        \
+    case 2 * NL:                                                                                          \
+        hipLaunchKernelGGL((crt_grouped_kernel<NL, false>), dim3(g.blocks), dim3(256), 0, st, items, g.d_start, g.n); \
+        break;                                                                                            \
+    case 2 * NL + 1:                                                                                      \
+        hipLaunchKernelGGL((crt_grouped_kernel<NL, true>), dim3(g.blocks), dim3(256), 0, st, items, g.d_start, g.n); \
+        break;
+        OZ2_NL(2) OZ2_NL(3) OZ2_NL(4) OZ2_NL(5) OZ2_NL(6)
+#undef OZ2_NL
         default: break;
     }
 }

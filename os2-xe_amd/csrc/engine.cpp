@@ -43,6 +43,10 @@ Engine::Engine(hipStream_t stream) : stream_(stream) {
         (void)hipFree(dtabs_);
         hip_check(e, "table upload");
     }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
+                                                 hipSuccess && cus > 0)
+        cus_ = cus;
 }
 
 Engine::~Engine() {

@@ -71,14 +71,18 @@ public:
     // Workspace a call would need (no allocation); used to pre-size before graph capture.
     size_t workspace_for(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx);
 
+    // shared with batch plans
+    const TabSet* device_tables() const { return dtabs_; }
+    const Plan& plan_for(int s, int64_t k);
+    k::Tile pick_tile(int64_t mb, int64_t nb, int planes) const;
+    size_t budget() const;
+    int compute_units() const { return cus_; }
+
 private:
     enum class Stage { Begin, Scale, Split, Gemm, Crt };
     void run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, double ai, const double* A, int64_t lda,
              const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s, bool cplx);
-    const Plan& plan_for(int s, int64_t k);
     void record(Stage st);
-    size_t budget() const;
-    k::Tile pick_tile(int64_t mb, int64_t nb, int planes) const;
     struct Layout;
     Layout layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch, int64_t mb,
                   int64_t nb) const;
@@ -101,6 +105,7 @@ private:
     std::vector<std::pair<Stage, size_t>> stages_;
     bool accumulate_ = false;
     int64_t calls_ = 0;
+    int cus_ = 256;
 };
 
 }  // namespace oz2

@@ -29,6 +29,9 @@
 // MFMA 32x32x32 i8 operand layout (checked on gfx950): lane l holds row
 // (l mod 32) of A / column (l mod 32) of B with 16 k-bytes at k = 16*(l/32);
 // accumulator v of lane l is C[8*(v/4) + 4*(l/32) + v%4][l mod 32].
+#include <algorithm>
+
+#include "grouped.hpp"
 #include "kernels.hpp"
 #include "math.hpp"
 
@@ -109,36 +112,41 @@ __device__ __forceinline__ void residue_epilogue(const GemmArgs& p, v16i (&acc)[
     }
 }
 
-template <int BM, int BN, int MINB>
-__global__ void __launch_bounds__(256, MINB) igemm_residue_kernel(GemmArgs p) {
+// tile (tm, tn) of plane `plane` inside a problem's grid: columns of 8 tile-rows,
+// walking tile-columns inside a group (tiles that run together share A and B rows)
+__device__ __forceinline__ void tile_coords(const GemmArgs& p, uint32_t bid, uint32_t& plane, uint32_t& tm,
+                                            uint32_t& tn) {
+    const uint32_t tiles = static_cast<uint32_t>(p.tiles_m) * static_cast<uint32_t>(p.tiles_n);
+    plane = bid / tiles;
+    const uint32_t tt = bid - plane * tiles;
+    const uint32_t G = 8;
+    const uint32_t per_group = G * p.tiles_n;
+    const uint32_t g = tt / per_group;
+    const uint32_t first = g * G;
+    const uint32_t gs = min(G, static_cast<uint32_t>(p.tiles_m) - first);
+    const uint32_t in = tt - g * per_group;
+    tm = first + in % gs;
+    tn = in / gs;
+}
+
+template <int BM, int BN>
+struct TileShape {
+    static constexpr int STAGE = (BM + BN) * KT;  // bytes
+    static constexpr int SMEM = NST * STAGE;
+};
+
+// one BM x BN macro tile of one plane: k loop and residue epilogue; smem holds NST stages
+template <int BM, int BN>
+__device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uint32_t tm, uint32_t tn, v4i* smem) {
     constexpr int WTM = BM / 2, WTN = BN / 2;
     constexpr int FM = WTM / 32, FN = WTN / 32;
     constexpr int LA = BM / 64, LB = BN / 64;  // 1 KB DMAs per wave per stage
-    constexpr int STAGE = (BM + BN) * KT;       // bytes
+    constexpr int STAGE = TileShape<BM, BN>::STAGE;
     constexpr int NR = 2 * (FM + FN);           // fragment reads per stage
     constexpr int NMF = 2 * FM * FN;            // MFMAs per stage
     constexpr int ND = LA + LB;                 // DMAs per stage
     constexpr int VM_AHEAD = (NST - 3) * ND;    // DMAs still in flight when a stage must have landed
     static_assert(WTM * WTN * 4 <= NST * STAGE, "epilogue staging must fit in the stage buffers");
-    __shared__ v4i smem[NST * STAGE / 16];
-
-    const uint32_t nblk = gridDim.x;
-    const uint32_t bid = xcd_remap(blockIdx.x, nblk);
-    const uint32_t tiles = static_cast<uint32_t>(p.tiles_m) * static_cast<uint32_t>(p.tiles_n);
-    const uint32_t plane = bid / tiles;
-    uint32_t tt = bid - plane * tiles;
-    // grouped order: columns of 8 tile-rows, walking tile-columns inside a group
-    uint32_t tm, tn;
-    {
-        const uint32_t G = 8;
-        const uint32_t per_group = G * p.tiles_n;
-        const uint32_t g = tt / per_group;
-        const uint32_t first = g * G;
-        const uint32_t gs = min(G, static_cast<uint32_t>(p.tiles_m) - first);
-        const uint32_t in = tt - g * per_group;
-        tm = first + in % gs;
-        tn = in / gs;
-    }
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -245,9 +253,83 @@ __global__ void __launch_bounds__(256, MINB) igemm_residue_kernel(GemmArgs p) {
                              static_cast<int64_t>(tn) * BN + wn * WTN, sbase + wave * (WTM * WTN), lane);
 }
 
+template <int BM, int BN, int MINB>
+__global__ void __launch_bounds__(256, MINB) igemm_residue_kernel(GemmArgs p) {
+    __shared__ v4i smem[TileShape<BM, BN>::SMEM / 16];
+    uint32_t plane, tm, tn;
+    tile_coords(p, xcd_remap(blockIdx.x, gridDim.x), plane, tm, tn);
+    gemm_tile<BM, BN>(p, plane, tm, tn, smem);
+}
+
+// Persistent grouped form: a fixed grid takes work items (problem, plane, tile)
+// from a counter until all `total` are done; items are numbered problem by
+// problem in the order the host listed them (largest first balances the tail).
+template <int BM, int BN, int MINB>
+__global__ void __launch_bounds__(256, MINB) igemm_grouped_kernel(const GemmArgs* __restrict__ items,
+                                                                  const uint32_t* __restrict__ start, uint32_t n,
+                                                                  uint32_t* __restrict__ counter) {
+    __shared__ v4i smem[TileShape<BM, BN>::SMEM / 16];
+    __shared__ uint32_t next;
+    const uint32_t total = start[n];
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t w = __builtin_amdgcn_readfirstlane(next);  // uniform: item fields stay in SGPRs
+        __syncthreads();  // also: the previous tile's epilogue is done with LDS
+        if (w >= total) break;
+        const uint32_t it = owner_of(start, n, w);
+        const GemmArgs& p = items[it];
+        uint32_t plane, tm, tn;
+        tile_coords(p, w - start[it], plane, tm, tn);
+        gemm_tile<BM, BN>(p, plane, tm, tn, smem);
+    }
+}
+
+__global__ void reset_counters(uint32_t* c, int n) {
+    if (static_cast<int>(threadIdx.x) < n) c[threadIdx.x] = 0;
+}
+
 }  // namespace
 
 int tile_rows(Tile t) { return t == Tile::T256 ? 256 : t == Tile::T128 ? 128 : 64; }
+
+// blocks of the persistent grouped GEMM per CU (LDS: 128 / 64 / 32 KB per block)
+static int tile_blocks_per_cu(Tile t) { return t == Tile::T256 ? 1 : t == Tile::T128 ? 2 : 4; }
+
+void prep_gemm(std::vector<GroupedLaunch>& out, const std::vector<GemmJob>& jobs) {
+    GroupedLaunch by[3];
+    for (const GemmJob& j : jobs) {
+        const uint32_t nblk = static_cast<uint32_t>(j.args.tiles_m) * j.args.tiles_n * j.args.nplanes;
+        if (nblk == 0 || j.args.klen <= 0) continue;
+        const int t = static_cast<int>(j.tile);
+        by[t].kind = t;
+        by[t].add(j.args, nblk);
+    }
+    for (GroupedLaunch& g : by)
+        if (g.n) out.push_back(std::move(g));
+}
+
+void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* counter, int cus) {
+    if (g.n == 0) return;
+    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, counter, 1);
+    const Tile t = static_cast<Tile>(g.kind);
+    const uint32_t grid = std::min<uint32_t>(g.blocks, static_cast<uint32_t>(cus * tile_blocks_per_cu(t)));
+    const GemmArgs* items = static_cast<const GemmArgs*>(g.d_items);
+    switch (t) {
+        case Tile::T256:
+            hipLaunchKernelGGL((igemm_grouped_kernel<256, 256, 1>), dim3(grid), dim3(256), 0, st, items, g.d_start, g.n,
+                               counter);
+            break;
+        case Tile::T128:
+            hipLaunchKernelGGL((igemm_grouped_kernel<128, 128, 2>), dim3(grid), dim3(256), 0, st, items, g.d_start, g.n,
+                               counter);
+            break;
+        case Tile::T64:
+            hipLaunchKernelGGL((igemm_grouped_kernel<64, 64, 4>), dim3(grid), dim3(256), 0, st, items, g.d_start, g.n,
+                               counter);
+            break;
+    }
+}
 
 void igemm_residue(hipStream_t st, Tile tile, const GemmArgs& a) {
     const uint32_t nblk = static_cast<uint32_t>(a.tiles_m) * a.tiles_n * a.nplanes;

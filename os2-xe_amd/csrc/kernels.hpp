@@ -10,6 +10,9 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include <vector>
+
+#include "grouped.hpp"
 #include "plan.hpp"
 
 namespace oz2 {
@@ -91,6 +94,30 @@ void crt(hipStream_t st, const CrtArgs& a);
 
 // ---- C = beta * C for k == 0 or alpha == 0 --------------------------------------
 void scale_c(hipStream_t st, double* C, int64_t ldc, int64_t m, int64_t n, double br, double bi, bool cplx);
+
+// ---- grouped launches over many problems (batched GEMMs) -------------------------
+// prep_* append launches (one per kernel variant the jobs need) whose blocks
+// run exactly the single-problem kernels' blocks; launch order = vector order.
+struct ScaleJob {
+    const double* X;
+    int64_t ldx, len, k;
+    bool kcontig, cplx;
+    int tk;
+    uint64_t* partial;    // scale_partial_elems(len, k, kcontig) entries (strided rows only)
+    int32_t* ex;
+};
+struct GemmJob {
+    Tile tile;
+    GemmArgs args;
+};
+void prep_scale(std::vector<GroupedLaunch>& out, const std::vector<ScaleJob>& jobs);
+void prep_split(std::vector<GroupedLaunch>& out, const std::vector<SplitArgs>& jobs);
+void prep_gemm(std::vector<GroupedLaunch>& out, const std::vector<GemmJob>& jobs);
+void prep_crt(std::vector<GroupedLaunch>& out, const std::vector<CrtArgs>& jobs);
+bool launch_scale_split_grouped(hipStream_t st, const GroupedLaunch& g);
+// persistent: min(blocks, cus x blocks per CU) workgroups pull tiles from *counter (reset first)
+void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* counter, int cus);
+void launch_crt_grouped(hipStream_t st, const GroupedLaunch& g);
 
 }  // namespace k
 }  // namespace oz2

@@ -171,4 +171,14 @@ void make_tabsets(TabSet* out) {
     }
 }
 
+int moduli_for_bits(int64_t k, int bits) {
+    for (int s = kMinModuli; s <= kMaxModuli; ++s) {
+        try {
+            if (make_plan(s, k).tk >= bits) return s;
+        } catch (const std::invalid_argument&) {
+        }
+    }
+    return kMaxModuli;
+}
+
 }  // namespace oz2

@@ -60,6 +60,8 @@ struct Plan {
 // Throws std::invalid_argument (s out of range, k < 1, s too small for k),
 // std::runtime_error (bad table) or std::logic_error (limb count).
 Plan make_plan(int s, int64_t k);
+// smallest s whose plan keeps >= bits per element; kMaxModuli if none does (hpp:693-700)
+int moduli_for_bits(int64_t k, int bits);
 
 // ---- constant tables read by the kernels -----------------------------------
 // One TabSet per s (2..24), built on the host and copied to device memory once

@@ -14,6 +14,7 @@
 // with lanes along rows, so no separate transpose pass is needed.
 #include <algorithm>
 
+#include "grouped.hpp"
 #include "kernels.hpp"
 #include "math.hpp"
 
@@ -34,12 +35,19 @@ __device__ __forceinline__ uint64_t absbits(const double* p) {
 }
 
 // One wave per row of op(X), row contiguous in memory: n doubles from base.
-__global__ void __launch_bounds__(256) scale_kcontig_kernel(const double* __restrict__ X, int64_t ldx, int64_t len,
-                                                            int64_t ndbl, int cplx, int tk, int32_t* __restrict__ ex) {
+struct ScaleKcItem {
+    const double* X;
+    int64_t ldx, len, ndbl;
+    int cplx, tk;
+    int32_t* ex;
+    uint32_t gx;
+};
+__device__ __forceinline__ void scale_kcontig_body(const ScaleKcItem& it, uint32_t bx) {
     const int lane = threadIdx.x & 63;
-    const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-    if (r >= len) return;
-    const double* row = X + (cplx ? 2 : 1) * r * ldx;
+    const int64_t r = static_cast<int64_t>(bx) * 4 + (threadIdx.x >> 6);
+    if (r >= it.len) return;
+    const int64_t ndbl = it.ndbl;
+    const double* row = it.X + (it.cplx ? 2 : 1) * r * it.ldx;
     uint64_t mx = 0;
     if ((reinterpret_cast<uintptr_t>(row) & 15) == 0) {
         const int64_t n2 = ndbl >> 1;
@@ -56,21 +64,28 @@ __global__ void __launch_bounds__(256) scale_kcontig_kernel(const double* __rest
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) mx = max(mx, shfl_xor_u64(mx, o));
-    if (lane == 0) ex[r] = exponent_from_maxbits(mx, tk);
+    if (lane == 0) it.ex[r] = exponent_from_maxbits(mx, it.tk);
 }
 
 // Rows strided in memory: block = 64 rows x 4 k-lanes over a k range; partial max per (k block, row).
-__global__ void __launch_bounds__(256) scale_kstrided_kernel(const double* __restrict__ X, int64_t ldx, int64_t len,
-                                                             int64_t k, int64_t kper, int cplx,
-                                                             uint64_t* __restrict__ partial) {
+struct ScaleKsItem {
+    const double* X;
+    int64_t ldx, len, k, kper;
+    int cplx;
+    uint64_t* partial;
+    uint32_t gx;
+};
+__device__ __forceinline__ void scale_kstrided_body(const ScaleKsItem& it, uint32_t bx, uint32_t by) {
     __shared__ uint64_t red[4][64];
     const int rl = threadIdx.x & 63, kl = threadIdx.x >> 6;
-    const int64_t r = static_cast<int64_t>(blockIdx.x) * 64 + rl;
-    const int64_t p0 = static_cast<int64_t>(blockIdx.y) * kper;
-    const int64_t p1 = min(k, p0 + kper);
+    const int64_t r = static_cast<int64_t>(bx) * 64 + rl;
+    const int64_t p0 = static_cast<int64_t>(by) * it.kper;
+    const int64_t p1 = min(it.k, p0 + it.kper);
+    const double* X = it.X;
+    const int64_t ldx = it.ldx;
     uint64_t mx = 0;
-    if (r < len) {
-        if (cplx) {
+    if (r < it.len) {
+        if (it.cplx) {
             for (int64_t p = p0 + kl; p < p1; p += 4) {
                 const double2 v = *reinterpret_cast<const double2*>(X + 2 * (r + p * ldx));
                 mx = max(mx, max(static_cast<uint64_t>(__double_as_longlong(v.x) & 0x7FFFFFFFFFFFFFFFll),
@@ -82,20 +97,32 @@ __global__ void __launch_bounds__(256) scale_kstrided_kernel(const double* __res
     }
     red[kl][rl] = mx;
     __syncthreads();
-    if (kl == 0 && r < len) {
+    if (kl == 0 && r < it.len) {
         mx = max(max(red[0][rl], red[1][rl]), max(red[2][rl], red[3][rl]));
-        partial[static_cast<int64_t>(blockIdx.y) * len + r] = mx;
+        it.partial[static_cast<int64_t>(by) * it.len + r] = mx;
     }
 }
 
-__global__ void scale_finalize_kernel(const uint64_t* __restrict__ partial, int64_t len, int nkb, int tk,
-                                      int32_t* __restrict__ ex) {
-    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= len) return;
+struct ScaleFinItem {
+    const uint64_t* partial;
+    int64_t len;
+    int nkb, tk;
+    int32_t* ex;
+    uint32_t gx;
+};
+__device__ __forceinline__ void scale_finalize_body(const ScaleFinItem& it, uint32_t bx) {
+    const int64_t r = static_cast<int64_t>(bx) * 256 + threadIdx.x;
+    if (r >= it.len) return;
     uint64_t mx = 0;
-    for (int b = 0; b < nkb; ++b) mx = max(mx, partial[static_cast<int64_t>(b) * len + r]);
-    ex[r] = exponent_from_maxbits(mx, tk);
+    for (int b = 0; b < it.nkb; ++b) mx = max(mx, it.partial[static_cast<int64_t>(b) * it.len + r]);
+    it.ex[r] = exponent_from_maxbits(mx, it.tk);
 }
+
+__global__ void __launch_bounds__(256) scale_kcontig_kernel(ScaleKcItem it) { scale_kcontig_body(it, blockIdx.x); }
+__global__ void __launch_bounds__(256) scale_kstrided_kernel(ScaleKsItem it) {
+    scale_kstrided_body(it, blockIdx.x, blockIdx.y);
+}
+__global__ void __launch_bounds__(256) scale_finalize_kernel(ScaleFinItem it) { scale_finalize_body(it, blockIdx.x); }
 
 constexpr int64_t kScaleRowsPerBlock = 64;
 
@@ -114,16 +141,16 @@ int64_t scale_kblocks(int64_t len, int64_t k) {
 // KCONTIG: row r is contiguous (X[p + r*ldx]); block = 256 chunks along the row(s).
 // !KCONTIG: rows strided (X[r + p*ldx]); block = 32 rows x 8 chunks (128 k).
 template <bool WIDE, bool KCONTIG>
-__global__ void __launch_bounds__(256) split_real_kernel(SplitArgs a, int lg_tpr) {
+__device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, uint32_t bx, uint32_t by) {
     const TabSet* __restrict__ tb = a.tab;
     int64_t r, ck;
     if (KCONTIG) {
         const int tpr = 1 << lg_tpr;  // threads per row in a block
-        r = static_cast<int64_t>(blockIdx.x) * (256 >> lg_tpr) + (threadIdx.x >> lg_tpr);
-        ck = static_cast<int64_t>(blockIdx.y) * tpr + (threadIdx.x & (tpr - 1));
+        r = static_cast<int64_t>(bx) * (256 >> lg_tpr) + (threadIdx.x >> lg_tpr);
+        ck = static_cast<int64_t>(by) * tpr + (threadIdx.x & (tpr - 1));
     } else {
-        r = static_cast<int64_t>(blockIdx.x) * 32 + (threadIdx.x & 31);
-        ck = static_cast<int64_t>(blockIdx.y) * 8 + (threadIdx.x >> 5);
+        r = static_cast<int64_t>(bx) * 32 + (threadIdx.x & 31);
+        ck = static_cast<int64_t>(by) * 8 + (threadIdx.x >> 5);
     }
     const int64_t nck = a.kp >> 4;
     if (r >= a.rows_pad || ck >= nck) return;
@@ -161,16 +188,16 @@ __global__ void __launch_bounds__(256) split_real_kernel(SplitArgs a, int lg_tpr
 // Complex split: three plane groups Re, Im (negated for conj), Re+Im, each
 // reduced to the symmetric range [-h, m-1-h].  4 consecutive k per thread.
 template <bool WIDE, bool KCONTIG, bool CONJ>
-__global__ void __launch_bounds__(256) split_cplx_kernel(SplitArgs a, int lg_tpr) {
+__device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, uint32_t bx, uint32_t by) {
     const TabSet* __restrict__ tb = a.tab;
     int64_t r, ck;
     if (KCONTIG) {
         const int tpr = 1 << lg_tpr;
-        r = static_cast<int64_t>(blockIdx.x) * (256 >> lg_tpr) + (threadIdx.x >> lg_tpr);
-        ck = static_cast<int64_t>(blockIdx.y) * tpr + (threadIdx.x & (tpr - 1));
+        r = static_cast<int64_t>(bx) * (256 >> lg_tpr) + (threadIdx.x >> lg_tpr);
+        ck = static_cast<int64_t>(by) * tpr + (threadIdx.x & (tpr - 1));
     } else {
-        r = static_cast<int64_t>(blockIdx.x) * 32 + (threadIdx.x & 31);
-        ck = static_cast<int64_t>(blockIdx.y) * 8 + (threadIdx.x >> 5);
+        r = static_cast<int64_t>(bx) * 32 + (threadIdx.x & 31);
+        ck = static_cast<int64_t>(by) * 8 + (threadIdx.x >> 5);
     }
     const int64_t nck = a.kp >> 2;
     if (r >= a.rows_pad || ck >= nck) return;
@@ -216,23 +243,70 @@ __global__ void __launch_bounds__(256) split_cplx_kernel(SplitArgs a, int lg_tpr
     }
 }
 
-template <typename K>
-void launch_split(hipStream_t st, K kernel, const SplitArgs& a, int elems_per_thread) {
-    const int64_t nck = a.kp / elems_per_thread;
-    dim3 grid, block(256);
-    int lg = 0;
+// split variants: bit 0 wide, 1 kcontig, 2 complex, 3 conj
+struct SplitItem {
+    SplitArgs a;
+    int lg;
+    uint32_t gx;
+};
+template <int V>
+__device__ __forceinline__ void split_body(const SplitItem& it, uint32_t bx, uint32_t by) {
+    constexpr bool W = V & 1, KC = (V >> 1) & 1, CX = (V >> 2) & 1, CJ = (V >> 3) & 1;
+    if constexpr (CX) split_cplx_body<W, KC, CJ>(it.a, it.lg, bx, by);
+    else split_real_body<W, KC>(it.a, it.lg, bx, by);
+}
+template <int V>
+__global__ void __launch_bounds__(256) split_kernel(SplitItem it) { split_body<V>(it, blockIdx.x, blockIdx.y); }
+
+int split_variant(const SplitArgs& a) {
+    return (a.wide ? 1 : 0) | (a.kcontig ? 2 : 0) | (a.cplx ? 4 : 0) | (a.cplx && a.conj ? 8 : 0);
+}
+
+// grid of one split: KCONTIG: rows x chunk groups (tpr threads per row); strided: 32 rows x 8 chunks
+SplitItem split_item(const SplitArgs& a, uint32_t& gy) {
+    const int64_t nck = a.kp / (a.cplx ? 4 : 16);
+    SplitItem it{a, 0, 0};
     if (a.kcontig) {
         int tpr = 1;
         while (tpr < 256 && tpr < nck) {
             tpr <<= 1;
-            ++lg;
+            ++it.lg;
         }
-        grid = dim3(static_cast<unsigned>((a.rows_pad + (256 / tpr) - 1) / (256 / tpr)),
-                    static_cast<unsigned>((nck + tpr - 1) / tpr));
+        it.gx = static_cast<uint32_t>((a.rows_pad + (256 / tpr) - 1) / (256 / tpr));
+        gy = static_cast<uint32_t>((nck + tpr - 1) / tpr);
     } else {
-        grid = dim3(static_cast<unsigned>((a.rows_pad + 31) / 32), static_cast<unsigned>((nck + 7) / 8));
+        it.gx = static_cast<uint32_t>((a.rows_pad + 31) / 32);
+        gy = static_cast<uint32_t>((nck + 7) / 8);
     }
-    hipLaunchKernelGGL(kernel, grid, block, 0, st, a, lg);
+    return it;
+}
+
+template <int V>
+void launch_split_v(hipStream_t st, const SplitItem& it, uint32_t gy) {
+    hipLaunchKernelGGL((split_kernel<V>), dim3(it.gx, gy), dim3(256), 0, st, it);
+}
+
+// ---- grouped forms: block b of the launch runs block (b - start[p]) of item p
+enum GroupKind { kScaleKc = 0, kScaleKs = 1, kScaleFin = 2, kSplitBase = 16 };
+
+template <typename Item, int K>
+__global__ void __launch_bounds__(256) grouped_scale_kernel(const Item* __restrict__ items,
+                                                            const uint32_t* __restrict__ start, uint32_t n) {
+    const uint32_t p = owner_of(start, n, blockIdx.x);
+    const Item& it = items[p];
+    const uint32_t l = blockIdx.x - start[p];
+    if constexpr (K == kScaleKc) scale_kcontig_body(it, l);
+    else if constexpr (K == kScaleKs) scale_kstrided_body(it, l % it.gx, l / it.gx);
+    else scale_finalize_body(it, l);
+}
+
+template <int V>
+__global__ void __launch_bounds__(256) grouped_split_kernel(const SplitItem* __restrict__ items,
+                                                            const uint32_t* __restrict__ start, uint32_t n) {
+    const uint32_t p = owner_of(start, n, blockIdx.x);
+    const SplitItem& it = items[p];
+    const uint32_t l = blockIdx.x - start[p];
+    split_body<V>(it, l % it.gx, l / it.gx);
 }
 
 }  // namespace
@@ -246,40 +320,97 @@ void scale(hipStream_t st, const double* X, int64_t ldx, int64_t len, int64_t k,
            uint64_t* partial, int32_t* ex) {
     if (len <= 0) return;
     if (kcontig) {
-        const int64_t ndbl = cplx ? 2 * k : k;
-        hipLaunchKernelGGL(scale_kcontig_kernel, dim3(static_cast<unsigned>((len + 3) / 4)), dim3(256), 0, st, X, ldx,
-                           len, ndbl, cplx ? 1 : 0, tk, ex);
+        const ScaleKcItem it{X, ldx, len, cplx ? 2 * k : k, cplx ? 1 : 0, tk, ex, static_cast<uint32_t>((len + 3) / 4)};
+        hipLaunchKernelGGL(scale_kcontig_kernel, dim3(it.gx), dim3(256), 0, st, it);
     } else {
         const int64_t nkb = scale_kblocks(len, k);
         const int64_t kper = (k + nkb - 1) / nkb;
-        const unsigned gx = static_cast<unsigned>((len + kScaleRowsPerBlock - 1) / kScaleRowsPerBlock);
-        hipLaunchKernelGGL(scale_kstrided_kernel, dim3(gx, static_cast<unsigned>(nkb)), dim3(256), 0, st, X, ldx, len,
-                           k, kper, cplx ? 1 : 0, partial);
-        hipLaunchKernelGGL(scale_finalize_kernel, dim3(static_cast<unsigned>((len + 255) / 256)), dim3(256), 0, st,
-                           partial, len, static_cast<int>(nkb), tk, ex);
+        const ScaleKsItem it{X, ldx, len, k, kper, cplx ? 1 : 0, partial,
+                             static_cast<uint32_t>((len + kScaleRowsPerBlock - 1) / kScaleRowsPerBlock)};
+        hipLaunchKernelGGL(scale_kstrided_kernel, dim3(it.gx, static_cast<unsigned>(nkb)), dim3(256), 0, st, it);
+        const ScaleFinItem fin{partial, len, static_cast<int>(nkb), tk, ex, static_cast<uint32_t>((len + 255) / 256)};
+        hipLaunchKernelGGL(scale_finalize_kernel, dim3(fin.gx), dim3(256), 0, st, fin);
     }
 }
 
 void split(hipStream_t st, const SplitArgs& a) {
-    if (!a.cplx) {
-        if (a.wide) {
-            if (a.kcontig) launch_split(st, split_real_kernel<true, true>, a, 16);
-            else launch_split(st, split_real_kernel<true, false>, a, 16);
+    uint32_t gy = 0;
+    const SplitItem it = split_item(a, gy);
+    switch (split_variant(a)) {
+#define OZ2_V(V) case V: launch_split_v<V>(st, it, gy); break;
+        OZ2_V(0) OZ2_V(1) OZ2_V(2) OZ2_V(3) OZ2_V(4) OZ2_V(5) OZ2_V(6) OZ2_V(7)
+        OZ2_V(12) OZ2_V(13) OZ2_V(14) OZ2_V(15)
+#undef OZ2_V
+        default: break;
+    }
+}
+
+void prep_scale(std::vector<GroupedLaunch>& out, const std::vector<ScaleJob>& jobs) {
+    GroupedLaunch kc, ks, fin;
+    kc.kind = kScaleKc;
+    ks.kind = kScaleKs;
+    fin.kind = kScaleFin;
+    for (const ScaleJob& j : jobs) {
+        if (j.len <= 0) continue;
+        if (j.kcontig) {
+            const ScaleKcItem it{j.X, j.ldx, j.len, j.cplx ? 2 * j.k : j.k, j.cplx ? 1 : 0, j.tk, j.ex,
+                                 static_cast<uint32_t>((j.len + 3) / 4)};
+            kc.add(it, it.gx);
         } else {
-            if (a.kcontig) launch_split(st, split_real_kernel<false, true>, a, 16);
-            else launch_split(st, split_real_kernel<false, false>, a, 16);
+            const int64_t nkb = scale_kblocks(j.len, j.k);
+            const int64_t kper = (j.k + nkb - 1) / nkb;
+            const ScaleKsItem it{j.X, j.ldx, j.len, j.k, kper, j.cplx ? 1 : 0, j.partial,
+                                 static_cast<uint32_t>((j.len + kScaleRowsPerBlock - 1) / kScaleRowsPerBlock)};
+            ks.add(it, it.gx * static_cast<uint32_t>(nkb));
+            const ScaleFinItem f{j.partial, j.len, static_cast<int>(nkb), j.tk, j.ex,
+                                 static_cast<uint32_t>((j.len + 255) / 256)};
+            fin.add(f, f.gx);
         }
-        return;
     }
-#define OZ2_CPLX(W, KC)                                                                         \
-    if (a.conj) launch_split(st, split_cplx_kernel<W, KC, true>, a, 4);                     \
-    else launch_split(st, split_cplx_kernel<W, KC, false>, a, 4);
-    if (a.wide) {
-        if (a.kcontig) { OZ2_CPLX(true, true) } else { OZ2_CPLX(true, false) }
-    } else {
-        if (a.kcontig) { OZ2_CPLX(false, true) } else { OZ2_CPLX(false, false) }
+    for (GroupedLaunch* g : {&kc, &ks, &fin})
+        if (g->n) out.push_back(std::move(*g));
+}
+
+void prep_split(std::vector<GroupedLaunch>& out, const std::vector<SplitArgs>& jobs) {
+    GroupedLaunch by[16];
+    for (const SplitArgs& a : jobs) {
+        if (a.rows_pad <= 0) continue;
+        uint32_t gy = 0;
+        const SplitItem it = split_item(a, gy);
+        const int v = split_variant(a);
+        by[v].kind = kSplitBase + v;
+        by[v].add(it, it.gx * gy);
     }
-#undef OZ2_CPLX
+    for (GroupedLaunch& g : by)
+        if (g.n) out.push_back(std::move(g));
+}
+
+bool launch_scale_split_grouped(hipStream_t st, const GroupedLaunch& g) {
+    if (g.n == 0) return true;
+    const dim3 grid(g.blocks), block(256);
+    switch (g.kind) {
+        case kScaleKc:
+            hipLaunchKernelGGL((grouped_scale_kernel<ScaleKcItem, kScaleKc>), grid, block, 0, st,
+                               static_cast<const ScaleKcItem*>(g.d_items), g.d_start, g.n);
+            return true;
+        case kScaleKs:
+            hipLaunchKernelGGL((grouped_scale_kernel<ScaleKsItem, kScaleKs>), grid, block, 0, st,
+                               static_cast<const ScaleKsItem*>(g.d_items), g.d_start, g.n);
+            return true;
+        case kScaleFin:
+            hipLaunchKernelGGL((grouped_scale_kernel<ScaleFinItem, kScaleFin>), grid, block, 0, st,
+                               static_cast<const ScaleFinItem*>(g.d_items), g.d_start, g.n);
+            return true;
+#define OZ2_V(V)                                                                                  \
+    case kSplitBase + V:                                                                          \
+        hipLaunchKernelGGL((grouped_split_kernel<V>), grid, block, 0, st,                         \
+                           static_cast<const SplitItem*>(g.d_items), g.d_start, g.n);             \
+        return true;
+        OZ2_V(0) OZ2_V(1) OZ2_V(2) OZ2_V(3) OZ2_V(4) OZ2_V(5) OZ2_V(6) OZ2_V(7)
+        OZ2_V(12) OZ2_V(13) OZ2_V(14) OZ2_V(15)
+#undef OZ2_V
+        default: return false;
+    }
 }
 
 }  // namespace k

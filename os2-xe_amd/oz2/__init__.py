@@ -100,6 +100,31 @@ _lib.oz2_profile_begin.argtypes = [_vp]
 _lib.oz2_profile_end.argtypes = [_vp, _P(StageTimes), _P(_i64)]
 _lib.oz2_unit_split.argtypes = [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp, _int, _int, _vp, _i64, _i64]
 _lib.oz2_unit_crt.argtypes = [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _int, _dbl, _dbl, _vp, _i64]
+
+
+class GemmDesc(ctypes.Structure):
+    """oz2_gemm_desc_t: one DGEMM of a batch (device addresses as ints)"""
+    _fields_ = [("ta", _int), ("tb", _int), ("m", _i64), ("n", _i64), ("k", _i64), ("alpha", _dbl), ("A", _vp),
+                ("lda", _i64), ("B", _vp), ("ldb", _i64), ("beta", _dbl), ("C", _vp), ("ldc", _i64),
+                ("num_moduli", _int)]
+
+
+_lib.oz2_batch_create.argtypes = [_vp, _int, _P(GemmDesc), _int, _P(_int), _int, _P(_vp)]
+_lib.oz2_batch_run.argtypes = [_vp]
+_lib.oz2_batch_moduli.argtypes = [_vp, _P(_int)]
+_lib.oz2_batch_tiles.argtypes = [_vp, _P(_int)]
+_lib.oz2_batch_info.argtypes = [_vp, _P(_sz), _P(_int), _P(_int)]
+_lib.oz2_batch_destroy.argtypes = [_vp]
+BATCH_GRAPH, BATCH_SELECT = 1, 2
+SEL_FEATURES, SEL_OUTPUTS, SEL_PARAMS = 48, 23, 8791
+_flt = ctypes.c_float
+_lib.oz2_selector_features.argtypes = [_vp, _int, _P(GemmDesc), _int, _P(_flt)]
+_lib.oz2_selector_choose.argtypes = [_vp, _int, _P(GemmDesc), _int, _P(_flt), _P(_int), _P(_int)]
+_lib.oz2_selector_set_weights.argtypes = [_vp, _P(_flt), _sz, _flt]
+_lib.oz2_selector_get_weights.argtypes = [_vp, _P(_flt), _sz, _P(_flt), _P(_sz)]
+_lib.oz2_mlp_forward.argtypes = [_vp, _vp, _int, _vp, _vp, _vp, _vp, _vp]
+_lib.oz2_mlp_backward.argtypes = [_vp, _vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _flt, _vp, _vp, _vp, _vp, _vp]
+_lib.oz2_mlp_adam.argtypes = [_vp, _vp, _vp, _vp, _vp, _int, _flt, _flt, _flt, _flt, _flt]
 _lib.oz2_last_error.restype = ctypes.c_char_p
 _lib.oz2_version.restype = ctypes.c_char_p
 
@@ -108,7 +133,9 @@ EXPORTS = [
     "oz2_precision_bits_complex", "oz2_moduli_for_bits", "oz2_set_workspace_limit", "oz2_set_resident_hint",
     "oz2_release_workspace", "oz2_reserve_workspace", "oz2_workspace_query", "oz2_stage_times", "oz2_last_tiling",
     "oz2_set_tuning", "oz2_get_tuning", "oz2_set_profiling", "oz2_profile_begin", "oz2_profile_end", "oz2_moduli",
-    "oz2_unit_split", "oz2_unit_crt",
+    "oz2_unit_split", "oz2_unit_crt", "oz2_batch_create", "oz2_batch_run", "oz2_batch_moduli", "oz2_batch_tiles",
+    "oz2_batch_info", "oz2_batch_destroy", "oz2_selector_features", "oz2_selector_choose",
+    "oz2_selector_set_weights", "oz2_selector_get_weights", "oz2_mlp_forward", "oz2_mlp_backward", "oz2_mlp_adam",
     "oz2_last_error", "oz2_version",
 ]
 
@@ -152,6 +179,65 @@ def unit_crt(R, ldr, mb, nb, D, E, num_moduli, alpha, beta, C, ldc, stream=None)
     """Kernel-level CRT + epilogue (the reference's detail::launch_crt in its unit test)."""
     _check(_lib.oz2_unit_crt(stream, _ptr(R), ldr, mb, nb, _ptr(D), _ptr(E), num_moduli, float(alpha), float(beta),
                              _ptr(C), ldc))
+
+
+def _descs(problems):
+    n = len(problems)
+    arr = (GemmDesc * max(1, n))()
+    for i, p in enumerate(problems):
+        arr[i] = GemmDesc(_TRANS.get(p.get("ta", "N"), -1), _TRANS.get(p.get("tb", "N"), -1), p["m"], p["n"],
+                          p["k"], float(p.get("alpha", 1.0)), _ptr(p["A"]), p["lda"], _ptr(p["B"]), p["ldb"],
+                          float(p.get("beta", 0.0)), _ptr(p.get("C")), p.get("ldc", max(1, p["m"])),
+                          int(p.get("num_moduli", 0)))
+    return arr
+
+
+class Batch:
+    """A plan for many DGEMMs (oz2_batch_create).  problems: dicts with keys ta, tb, m, n, k, alpha, A, lda,
+    B, ldb, beta, C, ldc and optional num_moduli (<= 0 or absent: chosen by the plan for `bits`).
+    The plan keeps the tensors it was given alive; run() recomputes from their current contents."""
+
+    def __init__(self, oz, problems, bits=53, tiles=None, graph=True, select=False):
+        self._oz = oz
+        self._keep = [(p["A"], p["B"], p["C"]) for p in problems]
+        n = len(problems)
+        arr = _descs(problems)
+        t = None
+        if tiles is not None:
+            t = (_int * max(1, n))(*tiles)
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_SELECT if select else 0)
+        self._b = _vp()
+        _check(_lib.oz2_batch_create(oz.handle, n, arr, int(bits), t, flags, ctypes.byref(self._b)))
+        self.count = n
+
+    def run(self):
+        _check(_lib.oz2_batch_run(self._b))
+
+    def moduli(self):
+        out = (_int * max(1, self.count))()
+        _check(_lib.oz2_batch_moduli(self._b, out))
+        return list(out)[:self.count]
+
+    def tiles(self):
+        out = (_int * max(1, self.count))()
+        _check(_lib.oz2_batch_tiles(self._b, out))
+        return list(out)[:self.count]
+
+    def info(self):
+        ws, launches, graph = _sz(0), _int(0), _int(0)
+        _check(_lib.oz2_batch_info(self._b, ctypes.byref(ws), ctypes.byref(launches), ctypes.byref(graph)))
+        return {"workspace_bytes": ws.value, "launches": launches.value, "graph": bool(graph.value)}
+
+    def close(self):
+        if getattr(self, "_b", None):
+            _lib.oz2_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def version():
@@ -281,3 +367,48 @@ class Ozaki2:
 
     def set_profiling(self, on):
         _check(_lib.oz2_set_profiling(self._h, int(bool(on))))
+
+    # -- learned moduli selector ------------------------------------------------------
+    def selector_features(self, problems, bits=53):
+        """list of problem dicts (as for Batch, C not needed) -> numpy array (n, 48)"""
+        import numpy as np
+        n = len(problems)
+        out = (_flt * max(1, n * SEL_FEATURES))()
+        _check(_lib.oz2_selector_features(self._h, n, _descs(problems), int(bits), out))
+        return np.ctypeslib.as_array(out)[:n * SEL_FEATURES].reshape(n, SEL_FEATURES).copy()
+
+    def selector_choose(self, problems, bits=53):
+        """-> (s per problem, probabilities (n, 23), learned flags)"""
+        import numpy as np
+        n = len(problems)
+        pr = (_flt * max(1, n * SEL_OUTPUTS))()
+        s = (_int * max(1, n))()
+        lf = (_int * max(1, n))()
+        _check(_lib.oz2_selector_choose(self._h, n, _descs(problems), int(bits), pr, s, lf))
+        return (list(s)[:n], np.ctypeslib.as_array(pr)[:n * SEL_OUTPUTS].reshape(n, SEL_OUTPUTS).copy(),
+                [bool(x) for x in list(lf)[:n]])
+
+    def selector_set_weights(self, weights, threshold=0.9):
+        import numpy as np
+        w = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
+        _check(_lib.oz2_selector_set_weights(self._h, w.ctypes.data_as(_P(_flt)), w.size, float(threshold)))
+
+    def selector_weights(self):
+        """-> (weights as float32 numpy array or None, threshold)"""
+        import numpy as np
+        w = np.zeros(SEL_PARAMS, dtype=np.float32)
+        tau, cnt = _flt(0), _sz(0)
+        _check(_lib.oz2_selector_get_weights(self._h, w.ctypes.data_as(_P(_flt)), w.size, ctypes.byref(tau),
+                                             ctypes.byref(cnt)))
+        return (w if cnt.value else None), tau.value
+
+    def mlp_forward(self, X, nb, W, H1=None, H2=None, Z=None, P=None):
+        _check(_lib.oz2_mlp_forward(self._h, _ptr(X), nb, _ptr(W), _ptr(H1), _ptr(H2), _ptr(Z), _ptr(P)))
+
+    def mlp_backward(self, X, nb, W, H1, H2, Z, Y, M, inv_norm, D1, D2, D3, G, loss=None):
+        _check(_lib.oz2_mlp_backward(self._h, _ptr(X), nb, _ptr(W), _ptr(H1), _ptr(H2), _ptr(Z), _ptr(Y), _ptr(M),
+                                     float(inv_norm), _ptr(D1), _ptr(D2), _ptr(D3), _ptr(G), _ptr(loss)))
+
+    def mlp_adam(self, W, G, M, V, step, lr, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0):
+        _check(_lib.oz2_mlp_adam(self._h, _ptr(W), _ptr(G), _ptr(M), _ptr(V), int(step), float(lr), float(beta1),
+                                 float(beta2), float(eps), float(wd)))

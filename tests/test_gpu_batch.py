@@ -1,0 +1,160 @@
+"""Batched DGEMMs (oz2_batch_*): one grouped launch sequence for many problems of
+different shapes, optionally replayed from a hipGraph.  Every problem's C must be
+bit-identical to oz2_dgemm with the same num_moduli (which the other GPU tests pin
+to the oracle), whatever the macro tiles, graph use or replay count; problems with
+k == 0 or alpha == 0 follow the C = beta*C rule (hpp:750-751) and beta == 0 never
+reads C (hpp:623)."""
+import numpy as np
+import pytest
+import torch
+
+import oz2
+from conftest import make_mat
+from gpu_cases import bits, flat, log, to_dev
+
+pytestmark = pytest.mark.gpu
+
+# (ta, tb, m, n, k, s, dist, alpha, beta)
+PROBLEMS = [
+    ("N", "N", 1, 1, 1, 8, "u11", 1.0, 0.0),
+    ("N", "N", 37, 53, 129, 12, "uniform", 1.5, 0.5),
+    ("T", "N", 300, 260, 200, 16, "phi", -2.0, 0.0),
+    ("N", "T", 64, 64, 64, 14, "u11", 1.0, 1.0),
+    ("T", "T", 129, 70, 1000, 16, "wide", 0.75, -0.25),
+    ("N", "N", 513, 129, 777, 20, "u11", 1.0, 0.0),
+    ("N", "T", 96, 200, 5000, 24, "wide", 1.0, 0.0),
+    ("T", "N", 256, 256, 256, 10, "uniform", 2.0, 3.0),
+    ("N", "N", 40, 30, 0, 16, "u11", 1.0, 0.5),      # k == 0: C = beta C
+    ("N", "N", 33, 17, 50, 16, "u11", 0.0, -1.0),    # alpha == 0: C = beta C
+    ("N", "N", 0, 17, 50, 16, "u11", 1.0, 0.0),      # empty: no-op
+    ("T", "T", 200, 1, 300, 18, "phi", 1.0, 0.0),
+]
+
+
+@pytest.fixture(scope="module")
+def oz():
+    h = oz2.Ozaki2()
+    yield h
+    h.close()
+
+
+def build(problems, seed=7):
+    rng = np.random.default_rng(seed)
+    out = []
+    for (ta, tb, m, n, k, s, dist, alpha, beta) in problems:
+        ar, ac = (m, k) if ta == "N" else (k, m)
+        br, bc = (k, n) if tb == "N" else (n, k)
+        lda, ldb, ldc = max(1, ar + 3), max(1, br + 1), max(1, m + 2)
+        A = make_mat(ar, ac, lda, dist, rng)
+        B = make_mat(br, bc, ldb, dist, rng)
+        C0 = make_mat(m, n, ldc, "u11", rng)
+        if beta == 0.0 and m and n:
+            C0[:m, :] = np.nan  # must not be read
+        out.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=alpha, beta=beta, s=s, lda=lda, ldb=ldb, ldc=ldc,
+                        A=to_dev(flat(A)), B=to_dev(flat(B)), C0=flat(C0)))
+    return out
+
+
+def reference(oz, p):
+    C = to_dev(p["C0"].copy())
+    oz.gemm(p["ta"], p["tb"], p["m"], p["n"], p["k"], p["alpha"], p["A"], p["lda"], p["B"], p["ldb"], p["beta"], C,
+            p["ldc"], p["s"])
+    return C
+
+
+def desc(p, C, s=None):
+    return dict(ta=p["ta"], tb=p["tb"], m=p["m"], n=p["n"], k=p["k"], alpha=p["alpha"], A=p["A"], lda=p["lda"],
+                B=p["B"], ldb=p["ldb"], beta=p["beta"], C=C, ldc=p["ldc"], num_moduli=p["s"] if s is None else s)
+
+
+def check_same(ref, got, what):
+    for i, (r, g) in enumerate(zip(ref, got)):
+        a, b = r.cpu().numpy(), g.cpu().numpy()
+        nbad = int(np.count_nonzero(bits(a) != bits(b)))
+        assert nbad == 0, f"{what}: problem {i} has {nbad} entries different from oz2_dgemm"
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_matches_single_calls(oz, graph):
+    log(f"batch of {len(PROBLEMS)} problems, graph={graph}")
+    ps = build(PROBLEMS)
+    ref = [reference(oz, p) for p in ps]
+    Cs = [to_dev(p["C0"].copy()) for p in ps]
+    b = oz2.Batch(oz, [desc(p, C) for p, C in zip(ps, Cs)], graph=graph)
+    assert b.moduli() == [p["s"] for p in ps]
+    info = b.info()
+    assert info["graph"] == graph and info["launches"] < 25, info
+    b.run()
+    torch.cuda.synchronize()
+    check_same(ref, Cs, "first run")
+    b.close()
+
+
+def test_forced_tiles_identical(oz):
+    ps = build(PROBLEMS[:8], seed=11)
+    ref = [reference(oz, p) for p in ps]
+    for t in (256, 128, 64):
+        log(f"batch with every macro tile {t}")
+        Cs = [to_dev(p["C0"].copy()) for p in ps]
+        b = oz2.Batch(oz, [desc(p, C) for p, C in zip(ps, Cs)], tiles=[t] * len(ps), graph=False)
+        assert all(x in (0, t) for x in b.tiles())
+        b.run()
+        torch.cuda.synchronize()
+        check_same(ref, Cs, f"tile {t}")
+        b.close()
+
+
+def test_graph_replay_reads_current_inputs(oz):
+    ps = build(PROBLEMS[1:6], seed=3)
+    Cs = [to_dev(p["C0"].copy()) for p in ps]
+    b = oz2.Batch(oz, [desc(p, C) for p, C in zip(ps, Cs)], graph=True)
+    for rep in range(3):
+        log(f"graph replay {rep}")
+        for p, C in zip(ps, Cs):
+            C.copy_(to_dev(p["C0"]))
+        if rep == 2:  # new contents of the same buffers
+            for p in ps:
+                p["A"].mul_(-0.5)
+                p["B"].add_(0.25)
+        b.run()
+        torch.cuda.synchronize()
+        ref = [reference(oz, p) for p in ps]
+        check_same(ref, Cs, f"replay {rep}")
+    b.close()
+
+
+def test_k_chunks_in_batch(oz):
+    probs = [("N", "T", 40, 24, 140000, 16, "u11", 1.0, 0.0), ("N", "N", 70, 66, 300, 12, "u11", 1.0, 0.5)]
+    ps = build(probs, seed=5)
+    ref = [reference(oz, p) for p in ps]
+    Cs = [to_dev(p["C0"].copy()) for p in ps]
+    b = oz2.Batch(oz, [desc(p, C) for p, C in zip(ps, Cs)], graph=True)
+    b.run()
+    torch.cuda.synchronize()
+    check_same(ref, Cs, "k-chunked")
+    b.close()
+
+
+def test_plan_chooses_moduli_for_bits(oz):
+    ps = build(PROBLEMS[1:4], seed=9)
+    Cs = [to_dev(p["C0"].copy()) for p in ps]
+    b = oz2.Batch(oz, [desc(p, C, s=0) for p, C in zip(ps, Cs)], bits=53, graph=False)
+    assert b.moduli() == [oz2.Ozaki2.moduli_for_bits(p["k"], 53) for p in ps]
+    b.close()
+
+
+def test_batch_argument_errors(oz):
+    ps = build(PROBLEMS[1:3], seed=1)
+    Cs = [to_dev(p["C0"].copy()) for p in ps]
+    d = [desc(p, C) for p, C in zip(ps, Cs)]
+    d[1]["ldc"] = 1
+    with pytest.raises(oz2.OZ2InvalidArgument, match="batch entry 1"):
+        oz2.Batch(oz, d)
+    d = [desc(p, C) for p, C in zip(ps, Cs)]
+    d[0]["num_moduli"] = 30
+    with pytest.raises(oz2.OZ2InvalidArgument, match="num_moduli"):
+        oz2.Batch(oz, d)
+    d = [desc(p, C) for p, C in zip(ps, Cs)]
+    d[0].update(k=140000, ldb=140000, num_moduli=2)  # s too small for k: make_plan's error (hpp:186), before any launch
+    with pytest.raises(oz2.OZ2InvalidArgument):
+        oz2.Batch(oz, d)

@@ -1,0 +1,173 @@
+"""Learned moduli selector on the GPU: the feature kernel, the MLP forward,
+backward and Adam kernels against the CPU oracle (oracle/selector.py, itself
+pinned by finite differences in tests/test_selector_oracle.py), training with
+those kernels, and the fallback rule."""
+import numpy as np
+import pytest
+import torch
+
+import oz2
+from conftest import make_mat
+from gpu_cases import flat, log, to_dev
+from oracle import selector as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def oz():
+    h = oz2.Ozaki2()
+    yield h
+    h.close()
+
+
+FEATURE_CASES = [  # ta, tb, m, n, k, dist
+    ("N", "N", 1, 1, 1, "u11"),
+    ("N", "T", 37, 53, 129, "wide"),
+    ("T", "N", 300, 130, 200, "phi"),
+    ("T", "T", 129, 70, 5000, "wide"),
+    ("N", "N", 200, 257, 3000, "ints"),
+    ("N", "T", 64, 64, 64, "sparse"),
+]
+
+
+def _problem(ta, tb, m, n, k, dist, rng):
+    ar, ac = (m, k) if ta == "N" else (k, m)
+    br, bc = (k, n) if tb == "N" else (n, k)
+    if dist == "sparse":
+        A = make_mat(ar, ac, ar, "u11", rng)
+        A[rng.random(A.shape) < 0.4] = 0
+        B = make_mat(br, bc, br, "u11", rng)
+        B[:, 0] = 0
+    else:
+        A = make_mat(ar, ac, ar, dist, rng, phi=2.0)
+        B = make_mat(br, bc, br, dist, rng, phi=2.0)
+    return flat(A), ar, flat(B), br
+
+
+def test_features_match_oracle(oz):
+    rng = np.random.default_rng(21)
+    for (ta, tb, m, n, k, dist) in FEATURE_CASES:
+        log(f"selector features {ta}{tb} m={m} n={n} k={k} {dist}")
+        A, lda, B, ldb = _problem(ta, tb, m, n, k, dist, rng)
+        d = dict(ta=ta, tb=tb, m=m, n=n, k=k, A=to_dev(A), lda=lda, B=to_dev(B), ldb=ldb)
+        for bits in (40, 53):
+            got = oz.selector_features([d], bits=bits)[0]
+            want = S.features(ta, tb, m, n, k, A, lda, B, ldb, bits)
+            assert np.allclose(got, want, rtol=2e-6, atol=1e-7), (np.nonzero(~np.isclose(got, want))[0], got, want)
+
+
+def test_features_of_a_batch_match_one_by_one(oz):
+    rng = np.random.default_rng(22)
+    ds = []
+    for (ta, tb, m, n, k, dist) in FEATURE_CASES:
+        A, lda, B, ldb = _problem(ta, tb, m, n, k, dist, rng)
+        ds.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, A=to_dev(A), lda=lda, B=to_dev(B), ldb=ldb))
+    all_f = oz.selector_features(ds, bits=53)
+    for i, d in enumerate(ds):
+        assert np.array_equal(all_f[i], oz.selector_features([d], bits=53)[0])
+
+
+def _rand_mlp(seed, nb):
+    rng = np.random.default_rng(seed)
+    W = rng.normal(0, 0.2, S.NPARAMS).astype(np.float32)
+    X = rng.normal(0, 1, (nb, S.F)).astype(np.float32)
+    Y = (rng.random((nb, S.O)) < 0.5).astype(np.float32)
+    M = (rng.random((nb, S.O)) < 0.85).astype(np.float32)
+    return W, X, Y, M
+
+
+def test_mlp_forward_matches_oracle(oz):
+    W, X, _, _ = _rand_mlp(1, 37)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    H1, H2 = torch.empty(37 * 64, device="cuda"), torch.empty(37 * 64, device="cuda")
+    Z, P = torch.empty(37 * 23, device="cuda"), torch.empty(37 * 23, device="cuda")
+    oz.mlp_forward(dev(X), 37, dev(W), H1, H2, Z, P)
+    torch.cuda.synchronize()
+    h1, h2, z, p = S.forward(X, W)
+    assert np.allclose(H1.cpu().numpy().reshape(37, 64), h1, rtol=1e-5, atol=1e-5)
+    assert np.allclose(H2.cpu().numpy().reshape(37, 64), h2, rtol=1e-5, atol=1e-5)
+    assert np.allclose(Z.cpu().numpy().reshape(37, 23), z, rtol=1e-5, atol=1e-5)
+    assert np.allclose(P.cpu().numpy().reshape(37, 23), p, rtol=1e-5, atol=1e-6)
+
+
+def test_mlp_gradient_matches_oracle(oz):
+    nb = 53
+    W, X, Y, M = _rand_mlp(2, nb)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    Xd, Wd = dev(X), dev(W)
+    H1, H2, D1, D2 = (torch.empty(nb * 64, device="cuda") for _ in range(4))
+    Z, D3, L = (torch.empty(nb * 23, device="cuda") for _ in range(3))
+    G = torch.empty(S.NPARAMS, device="cuda")
+    inv = 1.0 / float(M.sum())
+    oz.mlp_forward(Xd, nb, Wd, H1, H2, Z, None)
+    oz.mlp_backward(Xd, nb, Wd, H1, H2, Z, dev(Y), dev(M), inv, D1, D2, D3, G, L)
+    torch.cuda.synchronize()
+    g = S.backward(X, W, Y, M, inv)
+    got = G.cpu().numpy()
+    scale = np.abs(g).max()
+    assert np.max(np.abs(got - g)) <= 2e-5 * scale, np.max(np.abs(got - g)) / scale
+    assert np.isclose(float(L.sum()), S.loss(X, W, Y, M, inv), rtol=1e-5)
+
+
+def test_adam_matches_oracle(oz):
+    rng = np.random.default_rng(7)
+    W = rng.normal(0, 1, S.NPARAMS).astype(np.float32)
+    G = rng.normal(0, 1, S.NPARAMS).astype(np.float32)
+    M = rng.normal(0, 0.1, S.NPARAMS).astype(np.float32)
+    V = rng.random(S.NPARAMS).astype(np.float32) * 0.01
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    Wd, Gd, Md, Vd = dev(W), dev(G), dev(M), dev(V)
+    oz.mlp_adam(Wd, Gd, Md, Vd, 5, 1e-3, 0.9, 0.999, 1e-8, 1e-4)
+    torch.cuda.synchronize()
+    w, m, v = S.adam(W.astype(np.float64), G.astype(np.float64), M.astype(np.float64), V.astype(np.float64), 5, 1e-3,
+                     wd=1e-4)
+    assert np.allclose(Wd.cpu().numpy(), w, rtol=1e-6, atol=1e-6)
+    assert np.allclose(Md.cpu().numpy(), m, rtol=1e-5, atol=1e-7)
+    assert np.allclose(Vd.cpu().numpy(), v, rtol=1e-5, atol=1e-9)
+
+
+def test_training_with_the_kernels_fits_a_rule(oz):
+    """labels y_t = [x . u_t > 0]: 400 Adam steps with the library's kernels must cut the loss by 3x"""
+    rng = np.random.default_rng(8)
+    nb = 256
+    U = rng.normal(0, 1, (S.O, S.F))
+    X = rng.normal(0, 1, (2048, S.F)).astype(np.float32)
+    Y = (X @ U.T > 0).astype(np.float32)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    W = dev((rng.normal(0, 0.1, S.NPARAMS)).astype(np.float32))
+    Mm, Vv, G = torch.zeros_like(W), torch.zeros_like(W), torch.zeros_like(W)
+    H1, H2, D1, D2 = (torch.empty(nb * 64, device="cuda") for _ in range(4))
+    Z, D3, L = (torch.empty(nb * 23, device="cuda") for _ in range(3))
+    Xd, Yd = dev(X), dev(Y)
+    losses = []
+    for step in range(1, 401):
+        i0 = ((step - 1) * nb) % 2048
+        xb, yb = Xd[i0:i0 + nb].contiguous(), Yd[i0:i0 + nb].contiguous()
+        oz.mlp_forward(xb, nb, W, H1, H2, Z, None)
+        oz.mlp_backward(xb, nb, W, H1, H2, Z, yb, None, 1.0 / (nb * S.O), D1, D2, D3, G, L)
+        oz.mlp_adam(W, G, Mm, Vv, step, 3e-3)
+        if step in (1, 400):
+            torch.cuda.synchronize()
+            losses.append(float(L.sum()))
+    log(f"training loss {losses[0]:.4f} -> {losses[1]:.4f}")
+    assert losses[1] < losses[0] / 3
+
+
+def test_unloaded_selector_falls_back(oz):
+    rng = np.random.default_rng(9)
+    saved, tau = oz.selector_weights()
+    try:
+        oz.selector_set_weights(np.zeros(0, np.float32), 0.5)
+        ds = []
+        for (ta, tb, m, n, k, dist) in FEATURE_CASES[1:4]:
+            A, lda, B, ldb = _problem(ta, tb, m, n, k, dist, rng)
+            ds.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, A=to_dev(A), lda=lda, B=to_dev(B), ldb=ldb))
+        s, probs, learned = oz.selector_choose(ds, bits=53)
+        assert s == [oz2.Ozaki2.moduli_for_bits(d["k"], 53) for d in ds]
+        assert not any(learned) and not probs.any()
+        with pytest.raises(oz2.OZ2InvalidArgument):
+            oz.selector_set_weights(np.zeros(5, np.float32), 0.5)
+    finally:
+        if saved is not None:
+            oz.selector_set_weights(saved, tau)
