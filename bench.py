@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--workload", default="dgemm8192", choices=["dgemm8192", "zgemm4096", "wide4000", "ksplit"],
+                    help="dgemm8192 (default line), or one of the other measured configurations")
+    ap.add_argument("--batch-count", type=int, default=256)
     return ap.parse_args()
 
 
@@ -71,6 +75,201 @@ def accuracy(oz, torch, dA, dB, m, n, k, s):
     eu = err / ulp
     return {"ulp_max": float(eu.max()), "ulp_mean": float(eu.mean()), "cr_frac": float(np.mean(C == H + L)),
             "reference": "double-double GEMM on the GPU, same inputs, alpha=1 beta=0"}
+
+
+def make_batch(torch, count, seed):
+    """uneven batch: m, n, k log-uniform in [64, 4096]; even entries U[-1,1), odd entries
+    (1 + U[0,1)) * (+-1) * 2^e with e uniform in [-12, 12]; alpha = 1, beta = 0"""
+    import math
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    probs = []
+    for i in range(count):
+        m, n, k = (int(round(math.exp(rng.uniform(math.log(64), math.log(4096))))) for _ in range(3))
+        wide = i % 2 == 1
+
+        def gen(cnt):
+            if not wide:
+                return torch.rand(cnt, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+            e = torch.randint(-12, 13, (cnt,), device="cuda", generator=g).double()
+            sg = torch.randint(0, 2, (cnt,), device="cuda", generator=g).double() * 2 - 1
+            return (1 + torch.rand(cnt, dtype=torch.float64, device="cuda", generator=g)) * sg * torch.exp2(e)
+        probs.append(dict(ta="N", tb="N", m=m, n=n, k=k, alpha=1.0, beta=0.0, A=gen(m * k), lda=m, B=gen(k * n),
+                          ldb=k, C=torch.empty(m * n, dtype=torch.float64, device="cuda"), ldc=m, wide=wide))
+    return probs
+
+
+def batch_bench(oz, torch, count, reps=5, bits=53):
+    """learned-s grouped batch (hipGraph) vs the plain sequential fixed-s loop on the uneven mix"""
+    import oz2
+    from oz2 import check
+    probs = make_batch(torch, count, seed=5)
+    flops = sum(2.0 * p["m"] * p["n"] * p["k"] for p in probs)
+    refs = [check.dd_gemm("N", "N", p["m"], p["n"], p["k"], p["A"], p["lda"], p["B"], p["ldb"]) for p in probs]
+    torch.cuda.synchronize()
+
+    def max_ulp():
+        return max(float(check.ulp_errors(p["C"], hi, lo, p["m"], p["ldc"]).max()) for p, (hi, lo) in zip(probs, refs))
+
+    def plain(svals):
+        for p, sv in zip(probs, svals):
+            oz.gemm("N", "N", p["m"], p["n"], p["k"], 1.0, p["A"], p["lda"], p["B"], p["ldb"], 0.0, p["C"], p["ldc"], sv)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best
+
+    oz.set_profiling(False)
+    s_mfb = [oz2.Ozaki2.moduli_for_bits(p["k"], bits) for p in probs]
+    t_plain = timed(lambda: plain(s_mfb))
+    ulp_plain = max_ulp()
+    b = oz2.Batch(oz, [dict(p, num_moduli=0) for p in probs], bits=bits, graph=True, select=True)
+    t_batch = timed(b.run)
+    ulp_batch = max_ulp()
+    s_sel = b.moduli()
+    info = b.info()
+    # plain loop at the smallest uniform s whose max-ulp is no worse than the batch's
+    s_eq, t_eq, ulp_eq = None, None, None
+    for sv in range(min(s_mfb), 25):
+        plain([sv] * len(probs))
+        torch.cuda.synchronize()
+        u = max_ulp()
+        if u <= ulp_batch:
+            s_eq, ulp_eq = sv, u
+            t_eq = timed(lambda: plain([sv] * len(probs)))
+            break
+    b.close()
+    # the same batch with moduli_for_bits per problem: grouping and graph alone
+    b2 = oz2.Batch(oz, [dict(p, num_moduli=sv) for p, sv in zip(probs, s_mfb)], graph=True)
+    t_group = timed(b2.run)
+    b2.close()
+    oz.set_profiling(True)
+    out = {
+        "workload": f"{count} DGEMMs N/N, m n k log-uniform [64,4096], half U[-1,1) half (1+U)*2^e e in [-12,12], "
+                    f"alpha=1 beta=0, {bits}-bit target",
+        "fp64_flops": flops,
+        "plain_fixed_s": {"tflops": round(flops / t_plain / 1e12, 3), "ms": round(t_plain * 1e3, 3),
+                          "s": "moduli_for_bits(k, 53) per problem", "max_ulp": ulp_plain},
+        "grouped_graph_same_s": {"tflops": round(flops / t_group / 1e12, 3), "ms": round(t_group * 1e3, 3)},
+        "learned_grouped_graph": {"tflops": round(flops / t_batch / 1e12, 3), "ms": round(t_batch * 1e3, 3),
+                                  "max_ulp": ulp_batch, "mean_s": sum(s_sel) / len(s_sel),
+                                  "s_hist": {str(v): s_sel.count(v) for v in sorted(set(s_sel))},
+                                  "launches_per_run": info["launches"], "workspace_gib": round(info["workspace_bytes"] / 2 ** 30, 2)},
+        "speedup_vs_plain": round(t_plain / t_batch, 3),
+    }
+    if s_eq is not None:
+        out["plain_equal_max_ulp"] = {"s": s_eq, "max_ulp": ulp_eq, "tflops": round(flops / t_eq / 1e12, 3),
+                                      "ms": round(t_eq * 1e3, 3)}
+        out["speedup_at_equal_max_ulp"] = round(t_eq / t_batch, 3)
+    return out
+
+
+def zgemm_line(args, torch):
+    """ZGEMM 4096^3, op(A)=C, op(B)=N, s=16, alpha=(1,0.5), beta=(0.25,-1), lda=m+7, ldc=m+3 (3M form)"""
+    import oz2
+    from oz2 import check
+    m = n = k = 4096
+    s = 16
+    lda, ldb, ldc = k + 7, k, m + 3  # op(A) = A^H: A is k x m
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    dA = torch.rand(2 * lda * m, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    dB = torch.rand(2 * ldb * n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    dC0 = torch.rand(2 * ldc * n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    dC = dC0.clone()
+    oz = oz2.Ozaki2()
+    al, be = complex(1, 0.5), complex(0.25, -1)
+    for _ in range(args.warmup):
+        oz.zgemm("C", "N", m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, s)
+    torch.cuda.synchronize()
+    oz.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        oz.zgemm("C", "N", m, n, k, al, dA, lda, dB, ldb, be, dC, ldc, s)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    st, calls = oz.profile_end()
+    gemm_ms = st.gemm_ms / max(1, calls)
+    int8_ops = 2.0 * m * n * k * 3 * s
+    achieved = int8_ops / (gemm_ms * 1e-3) / 1e12
+    out = {"metric": "FP64-equiv TFLOP/s at stated max-ulp: ZGEMM 4096^3 s=16 (8mnk per call)",
+           "value": round(8.0 * m * n * k / (ms * 1e-3) / 1e12, 3), "unit": "TFLOP/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int8 (3M: 3s INT8 GEMMs; f64 in/out)",
+           "data": "synthetic U[-1,1) re and im, HBM-resident",
+           "config": {"workload": "ZGEMM 4096x4096x4096 op(A)=C op(B)=N s=16 alpha=(1,0.5) beta=(0.25,-1) "
+                                  "lda=k+7 ldc=m+3", "tk": oz2.Ozaki2.precision_bits_complex(k, s),
+                      "tiling": oz.last_tiling().as_dict()},
+           "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
+                        "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
+                        "kernel": "igemm_residue_kernel (2*m*n*k*3s int8 ops per launch)", "gemm_ms": round(gemm_ms, 4)},
+           "stage_ms": {kk: round(v / max(1, calls), 4) for kk, v in st.as_dict().items()}}
+    # accuracy with alpha=1, beta=0 against double-double (complex)
+    dC1 = torch.empty(2 * ldc * n, dtype=torch.float64, device="cuda")
+    oz.zgemm("C", "N", m, n, k, 1.0, dA, lda, dB, ldb, 0.0, dC1, ldc, s)
+    hi = torch.empty(2 * m * n, dtype=torch.float64, device="cuda")
+    lo = torch.empty_like(hi)
+    rc = check.lib().oz2chk_dd_gemm(None, 2, 0, m, n, k, dA.data_ptr(), lda, dB.data_ptr(), ldb, hi.data_ptr(),
+                                    lo.data_ptr(), 1)
+    assert rc == 0
+    torch.cuda.synchronize()
+    Cv = dC1.view(n, ldc, 2)[:, :m, :].reshape(-1)
+    e = check.ulp_errors(Cv, hi, lo, 2 * m * n, 2 * m * n)
+    out["accuracy"] = {"ulp_max": float(e.max()), "ulp_mean": float(e.mean()),
+                       "cr_frac": float((Cv == hi + lo).double().mean()),
+                       "reference": "double-double ZGEMM on the GPU, per real component, alpha=1 beta=0"}
+    oz.close()
+    return out
+
+
+def selector_line(args, torch, which):
+    """wide-exponent DGEMM (4000x8000x4000) or k-split 2048x2048x200000, entries (1+U)*(+-1)*2^e, e in [-20,20]:
+    learned s vs moduli_for_bits(k, 53), each with its max-ulp against double-double"""
+    import oz2
+    from oz2 import check
+    m, k, n = (4000, 8000, 4000) if which == "wide4000" else (2048, 200000, 2048)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+
+    def gen(cnt):
+        e = torch.randint(-20, 21, (cnt,), device="cuda", generator=g).double()
+        sg = torch.randint(0, 2, (cnt,), device="cuda", generator=g).double() * 2 - 1
+        return (1 + torch.rand(cnt, dtype=torch.float64, device="cuda", generator=g)) * sg * torch.exp2(e)
+    dA, dB = gen(m * k), gen(k * n)
+    dC = torch.empty(m * n, dtype=torch.float64, device="cuda")
+    oz = oz2.Ozaki2()
+    hi, lo = check.dd_gemm("N", "N", m, n, k, dA, m, dB, k)
+    desc = dict(ta="N", tb="N", m=m, n=n, k=k, A=dA, lda=m, B=dB, ldb=k)
+    s_sel, probs, learned = oz.selector_choose([desc], bits=53)
+    res = {}
+    for name, sv in (("moduli_for_bits", oz2.Ozaki2.moduli_for_bits(k, 53)), ("learned", s_sel[0])):
+        oz.gemm("N", "N", m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, sv)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            oz.gemm("N", "N", m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, sv)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        met = check.metrics(dC, hi, lo, m, m)
+        res[name] = {"s": sv, "ms": round(min(ts) * 1e3, 3), "tflops": round(2.0 * m * n * k / min(ts) / 1e12, 3),
+                     **met, "tiling": oz.last_tiling().as_dict()}
+    res["learned"]["selector_confident"] = bool(learned[0])
+    oz.close()
+    return {"metric": f"FP64-equiv TFLOP/s at stated max-ulp: DGEMM {m}x{n}x{k} wide exponents",
+            "value": res["learned"]["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8 (INT8 MFMA; f64 in/out)",
+            "data": "synthetic (1+U[0,1))*(+-1)*2^e, e uniform integer in [-20,20], HBM-resident",
+            "config": {"workload": f"DGEMM m={m} n={n} k={k} N/N alpha=1 beta=0, 53-bit target"}, "runs": res}
 
 
 def cpu_baseline(m, n, k, s):
@@ -105,6 +304,12 @@ def main():
     import torch
     import oz2
 
+    if args.workload == "zgemm4096":
+        print(json.dumps(zgemm_line(args, torch)))
+        return
+    if args.workload in ("wide4000", "ksplit"):
+        print(json.dumps(selector_line(args, torch, args.workload)))
+        return
     m, n, k, s = args.m, args.n, args.k, args.s
     alpha, beta = 1.5, 0.5
     g = torch.Generator(device="cuda")
@@ -138,7 +343,7 @@ def main():
     stage = {kk: v / max(1, calls) for kk, v in st.as_dict().items()}
 
     out = {
-        "metric": "FP64-equiv TFLOP/s at stated max-ulp: DGEMM 8192^3 s=16",
+        "metric": "FP64-equiv TFLOP/s at stated max-ulp: DGEMM 8192^3 s=16, uneven batch mix",
         "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int8 (INT8 MFMA, int32 accumulate; f64 in/out)", "data": "synthetic U[-1,1) seeded, HBM-resident",
@@ -171,6 +376,11 @@ def main():
                     ts.append(time.perf_counter() - a)
                 sweep[f"s{sv}_{ta}{tb}"] = round(flops / min(ts) / 1e12, 2)
         out["sweep_tflops"] = sweep
+    if not args.no_batch:
+        del dA, dB, dC, dC0
+        torch.cuda.empty_cache()
+        oz.release_workspace()
+        out["batch"] = batch_bench(oz, torch, args.batch_count)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, n, k, s)
     oz.close()

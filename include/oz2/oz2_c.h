@@ -85,8 +85,8 @@ int oz2_moduli(int num_moduli, uint16_t* out);
 /* ---- Batches of DGEMMs (no reference counterpart: the reference runs one GEMM per call) ----
  * A plan holds a list of DGEMMs, each with the full oz2_dgemm semantics (op, alpha, beta, leading
  * dimensions, num_moduli).  num_moduli > 0 fixes a problem's moduli; num_moduli <= 0 lets the plan
- * choose: the learned selector when OZ2_BATCH_SELECT is set (falling back to moduli_for_bits when it
- * is not confident), else moduli_for_bits(k, bits).  The plan owns one workspace for all problems and
+ * choose: the learned selector when OZ2_BATCH_SELECT is set (oz2_selector_choose), else
+ * moduli_for_bits(k, bits).  The plan owns one workspace for all problems and
  * a grouped launch sequence (scale, split, one persistent INT8 GEMM per macro-tile shape, CRT: about
  * ten launches for any count); with OZ2_BATCH_GRAPH that sequence is captured in a hipGraph when the
  * plan is created and oz2_batch_run replays it.  Buffers are bound at creation: running the plan
@@ -123,8 +123,9 @@ int oz2_batch_destroy(oz2_batch b);
  * Features: 48 floats per problem from sampled exponent gaps of op(A) rows / op(B) columns plus log2 k
  * and the bit target ("b bits": every entry within 2^(53-b) ulp of the exact product).  The network
  * (48 -> 64 -> 64 -> 23, FP32, OZ2_SEL_PARAMS floats) gives for s = 2..24 the probability that s moduli
- * reach the target; the choice is the smallest s whose probability reaches the threshold, else
- * moduli_for_bits(k, bits).  num_moduli in the descriptors is ignored here.  These calls synchronize
+ * reach the target; the choice is the smallest admissible s whose probability reaches the trained
+ * threshold, else the largest admissible s; moduli_for_bits(k, bits) when no weights are loaded.
+ * num_moduli in the descriptors is ignored here.  These calls synchronize
  * the handle's stream; out arrays are host memory. */
 enum { OZ2_SEL_FEATURES = 48, OZ2_SEL_OUTPUTS = 23, OZ2_SEL_PARAMS = 8791 };
 int oz2_selector_features(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, float* features);

@@ -161,10 +161,49 @@ public:
 
     oz2_handle native_handle() const { return h_; }
 
+    // New in this library (no reference counterpart): the learned choice of num_moduli for one
+    // problem and a bit target (moduli_for_bits when no selector weights are loaded).
+    int select_moduli(Trans ta, Trans tb, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda,
+                      const double* B, int64_t ldb, int bits = 53) {
+        oz2_gemm_desc_t d{detail::code(ta), detail::code(tb), m, n, k, 1.0, A, lda, B, ldb, 0.0, nullptr, 1, 0};
+        int s = 0;
+        detail::check(oz2_selector_choose(h_, 1, &d, bits, nullptr, &s, nullptr));
+        return s;
+    }
+
 private:
     oz2_handle h_ = nullptr;
     Tuning tune_{};
     TileInfo tiling_{};
+};
+
+// A plan for many DGEMMs (oz2_batch_*): one grouped launch sequence, optionally replayed from a
+// hipGraph; every problem's C equals Ozaki2::gemm with the same num_moduli.  Descriptors with
+// num_moduli <= 0 get the selector's choice (select = true) or moduli_for_bits(k, bits).
+class Batch {
+public:
+    Batch(Ozaki2& oz, const std::vector<oz2_gemm_desc_t>& descs, int bits = 53, bool graph = true,
+          bool select = false, const std::vector<int>& tiles = {}) {
+        const int flags = (graph ? OZ2_BATCH_GRAPH : 0) | (select ? OZ2_BATCH_SELECT : 0);
+        detail::check(oz2_batch_create(oz.native_handle(), static_cast<int>(descs.size()), descs.data(), bits,
+                                       tiles.empty() ? nullptr : tiles.data(), flags, &b_));
+        n_ = static_cast<int>(descs.size());
+    }
+    ~Batch() {
+        if (b_) (void)oz2_batch_destroy(b_);
+    }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+    void run() { detail::check(oz2_batch_run(b_)); }
+    std::vector<int> moduli() const {
+        std::vector<int> s(n_);
+        detail::check(oz2_batch_moduli(b_, s.data()));
+        return s;
+    }
+
+private:
+    oz2_batch b_ = nullptr;
+    int n_ = 0;
 };
 
 }  // namespace oz2

@@ -86,15 +86,16 @@ OZ2_HD void scaled_mag(uint64_t bits, int e, uint64_t& hi, uint64_t& lo) {
 }
 
 // Offset-binary u = 2^OFF + x (OFF = 62 narrow, 104 wide; x = +-RNE(|v| 2^e)) as
-// 21-bit limbs a[0..4].  zero: the element is outside the matrix or its row is
-// non-finite, giving x = 0.
+// limbs: wide: five 21-bit limbs; narrow (u <= 2^63): five 13-bit limbs (the
+// top one <= 2^11), so that the limb sum of limb_residue stays below 2^23.
+// zero: the element is outside the matrix or its row is non-finite, giving x = 0.
 template <bool WIDE>
 OZ2_HD void to_limbs(uint64_t bits, int e, bool zero, uint32_t (&a)[5]) {
     uint64_t hi = 0, lo = 0;
     if (!zero) scaled_mag(bits, e, hi, lo);
     const bool neg = (bits >> 63) != 0 && !zero;
-    uint64_t uh, ul;
     if (WIDE) {  // mag <= 2^100
+        uint64_t uh, ul;
         if (neg) {
             ul = 0 - lo;
             uh = (1ull << 40) - hi - (lo != 0 ? 1 : 0);
@@ -102,35 +103,47 @@ OZ2_HD void to_limbs(uint64_t bits, int e, bool zero, uint32_t (&a)[5]) {
             ul = lo;
             uh = (1ull << 40) + hi;
         }
-    } else {  // mag <= 2^62
-        uh = 0;
-        ul = neg ? (1ull << 62) - lo : (1ull << 62) + lo;
-    }
-    constexpr uint64_t M21 = 0x1FFFFF;
-    a[0] = static_cast<uint32_t>(ul & M21);
-    a[1] = static_cast<uint32_t>((ul >> 21) & M21);
-    if (WIDE) {
+        constexpr uint64_t M21 = 0x1FFFFF;
+        a[0] = static_cast<uint32_t>(ul & M21);
+        a[1] = static_cast<uint32_t>((ul >> 21) & M21);
         a[2] = static_cast<uint32_t>((ul >> 42) & M21);
         a[3] = static_cast<uint32_t>(((ul >> 63) | (uh << 1)) & M21);
         a[4] = static_cast<uint32_t>((uh >> 20) & M21);
-    } else {
-        a[2] = static_cast<uint32_t>(ul >> 42);  // u <= 2^63: at most 22 bits
-        a[3] = a[4] = 0;
+    } else {  // mag <= 2^62, u <= 2^63
+        const uint64_t ul = neg ? (1ull << 62) - lo : (1ull << 62) + lo;
+        constexpr uint64_t M13 = 0x1FFF;
+        a[0] = static_cast<uint32_t>(ul & M13);
+        a[1] = static_cast<uint32_t>((ul >> 13) & M13);
+        a[2] = static_cast<uint32_t>((ul >> 26) & M13);
+        a[3] = static_cast<uint32_t>((ul >> 39) & M13);
+        a[4] = static_cast<uint32_t>(ul >> 52);
     }
 }
 
-// (x + h) mod m in [0, m) for modulus index t (m <= 255) from the limbs of u = 2^OFF + x:
-// sum a_j (2^(21 j) mod m) + (h - 2^OFF mod m) stays below 2^32, then mod32.
+// (x + h) mod m in [0, m) for modulus index t (m <= 255) from the limbs of u = 2^OFF + x.
+// wide: v = a_0 + sum a_j (2^(21 j) mod m) + (h - 2^104 mod m) < 2^32, then mod32.
+// narrow: v = a_0 + sum a_j (2^(13 j) mod m) + (h - 2^62 mod m) < 2^23, accumulated
+// doubled (2v < 2^24, constants pre-doubled) so that q = mulhi24(2v, mu) =
+// floor(v mu / 2^31) is the Barrett quotient (floor(v/m) or one less) and
+// r = (2v - q 2m) / 2 lies in [0, 2m).
 template <bool WIDE>
 OZ2_HD uint32_t limb_residue(const uint32_t (&a)[5], const TabSet* __restrict__ tb, int t) {
-    uint32_t v = a[0] + tb->off[WIDE ? 1 : 0][t];
-    v += mul24(a[1], tb->c21[t][0]);
-    v += mul24(a[2], tb->c21[t][1]);
     if (WIDE) {
+        uint32_t v = a[0] + tb->off[1][t];
+        v += mul24(a[1], tb->c21[t][0]);
+        v += mul24(a[2], tb->c21[t][1]);
         v += mul24(a[3], tb->c21[t][2]);
         v += mul24(a[4], tb->c21[t][3]);
+        return mod32(v, tb->m[t], tb->mu[t], tb->c16[t]);
     }
-    return mod32(v, tb->m[t], tb->mu[t], tb->c16[t]);
+    uint32_t v2 = 2 * a[0] + tb->off2[t];
+    v2 += mul24(a[1], tb->c13x2[t][0]);
+    v2 += mul24(a[2], tb->c13x2[t][1]);
+    v2 += mul24(a[3], tb->c13x2[t][2]);
+    v2 += mul24(a[4], tb->c13x2[t][3]);
+    const uint32_t q = mulhi24(v2, tb->mu[t]);
+    const uint32_t r = (v2 - mul24(q, tb->m2[t])) >> 1;
+    return umin(r, r - tb->m[t]);
 }
 
 // int8 byte of the symmetric residue of x for modulus index t (m = 256 only at t = 0)

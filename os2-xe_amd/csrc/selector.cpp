@@ -103,17 +103,22 @@ std::vector<int> Selector::choose(hipStream_t st, const std::vector<SelProblem>&
         const int64_t k = std::max<int64_t>(probs[i].k, 1);
         s[i] = moduli_for_bits(k, probs[i].bits);
         if (!loaded()) continue;
+        // smallest admissible s the network is confident about; when it is confident about
+        // none, the most accurate admissible one
+        int pick = 0, largest = 0;
         for (int t = 0; t < kSelOutputs; ++t) {
             const int si = t + kMinModuli;
-            if (pr[i * kSelOutputs + t] < tau_) continue;
             try {
                 (void)make_plan(si, k);  // s must admit this k
             } catch (const std::invalid_argument&) {
                 continue;
             }
-            s[i] = si;
+            largest = si;
+            if (!pick && pr[i * kSelOutputs + t] >= tau_) pick = si;
+        }
+        if (largest) {
+            s[i] = pick ? pick : largest;
             if (learned) (*learned)[i] = true;
-            break;
         }
     }
     return s;

@@ -59,8 +59,8 @@ public:
     // features (nprob x 48) and output probabilities (nprob x 23) on the host; synchronizes `st`
     void evaluate(hipStream_t st, const std::vector<SelProblem>& probs, std::vector<float>* feat,
                   std::vector<float>* prob);
-    // smallest s whose probability reaches the threshold and that admits k; the fallback
-    // (moduli_for_bits(k, bits)) when none does or no weights are loaded
+    // smallest admissible s whose probability reaches the threshold, else the largest
+    // admissible s; moduli_for_bits(k, bits) when no weights are loaded
     std::vector<int> choose(hipStream_t st, const std::vector<SelProblem>& probs, std::vector<bool>* learned = nullptr);
 
 private:

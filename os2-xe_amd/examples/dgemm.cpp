@@ -45,6 +45,14 @@ int main(int argc, char** argv) {
 
     if (hipMemcpy(hC.data(), dC, hC.size() * 8, hipMemcpyDeviceToHost)) return 1;
     std::printf("C[0] = %.17g\n", hC[0]);
+
+    // the same product twice more as a batch (two problems sharing A and B): num_moduli = 0
+    // lets the learned selector choose from the data; the plan is replayed from a hipGraph
+    const oz2_gemm_desc_t d{OZ2_TRANS_N, OZ2_TRANS_N, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, 0};
+    oz2::Batch batch(oz, {d, d}, 53, /*graph=*/true, /*select=*/true);
+    batch.run();
+    if (hipStreamSynchronize(stream)) return 1;
+    std::printf("batch: learned s = %d (moduli_for_bits: %d)\n", batch.moduli()[0], s);
     (void)hipFree(dA);
     (void)hipFree(dB);
     (void)hipFree(dC);

@@ -12,9 +12,10 @@
 2. Training: the 48-64-64-23 MLP is trained with the library's own HIP
    kernels (oz2_mlp_forward / oz2_mlp_backward / oz2_mlp_adam) on masked binary
    cross-entropy; torch tensors only hold device memory.
-3. Threshold: the smallest probability threshold whose choices (smallest s with
-   p_s >= threshold) meet the target on every held-out example, capped to
-   [0.5, 0.995].
+3. Threshold: the smallest probability threshold whose choices (the library's
+   rule: smallest admissible s with p_s >= threshold, else the largest
+   admissible s) meet the target on at least 99.5 % of held-out examples,
+   searched in [0.5, 0.999].
 4. Output in DIR: dataset.npz, selector_weights.inc (the form compiled into
    liboz2.so: copy it to os2-xe_amd/csrc/) and report.json.
 """
@@ -89,7 +90,7 @@ def admissible(k):
 
 
 def make_problem(rng):
-    m, n = log_uniform_int(rng, 16, 512), log_uniform_int(rng, 16, 512)
+    m, n = log_uniform_int(rng, 16, 2048), log_uniform_int(rng, 16, 2048)
     k = log_uniform_int(rng, 16, 16384)
     ta, tb = rng.choice(["N", "T"]), rng.choice(["N", "T"])
     da, db = rng.choice(DISTS), rng.choice(DISTS)
@@ -150,7 +151,7 @@ def init_weights(rng):
     return np.concatenate(out)
 
 
-def train(oz, X, Y, M, rng, steps=4000, batch=512, lr=3e-3, wd=1e-5):
+def train(oz, X, Y, M, rng, steps=20000, batch=1024, lr=2e-3, wd=1e-5):
     dev = "cuda"
     W = torch.from_numpy(init_weights(rng)).to(dev)
     Mm = torch.zeros_like(W)
@@ -175,7 +176,7 @@ def train(oz, X, Y, M, rng, steps=4000, batch=512, lr=3e-3, wd=1e-5):
         oz.mlp_backward(xb, nb, W, H1, H2, Z, yb, mb, 1.0 / norm, D1, D2, D3, G, L)
         cur_lr = lr * (0.5 * (1 + math.cos(math.pi * step / steps)) * 0.95 + 0.05)
         oz.mlp_adam(W, G, Mm, Vv, step, cur_lr, 0.9, 0.999, 1e-8, wd)
-        if step % 500 == 0 or step == 1:
+        if step % 2000 == 0 or step == 1:
             torch.cuda.synchronize()
             log(f"  step {step}: batch loss {float(L.sum()):.4f} ({time.time() - t0:.1f} s)")
     torch.cuda.synchronize()
@@ -192,19 +193,18 @@ def predict(oz, W, X):
     return P.cpu().numpy().reshape(len(X), 23)
 
 
-def choose(P, M, k_of_row, tau, fallback):
-    """smallest admissible s with p >= tau, else the fallback"""
-    out = np.array(fallback, copy=True)
+def choose(P, M, tau):
+    """the library's rule: smallest admissible s with p >= tau, else the largest admissible s"""
+    out = np.zeros(len(P), dtype=int)
     for r in range(len(P)):
-        for t in range(23):
-            if M[r, t] > 0 and P[r, t] >= tau:
-                out[r] = t + SMIN
-                break
+        adm = [t for t in range(23) if M[r, t] > 0]
+        ok = [t for t in adm if P[r, t] >= tau]
+        out[r] = (ok[0] if ok else adm[-1]) + SMIN
     return out
 
 
 def evaluate(P, Y, M, fallback, tau):
-    s = choose(P, M, None, tau, fallback)
+    s = choose(P, M, tau)
     met = np.array([Y[r, s[r] - SMIN] > 0 for r in range(len(s))])
     fb_met = np.array([Y[r, fallback[r] - SMIN] > 0 for r in range(len(s))])
     opt = np.array([next((t + SMIN for t in range(23) if Y[r, t] > 0 and M[r, t] > 0), SMAX) for r in range(len(s))])
@@ -226,9 +226,9 @@ def write_inc(path, W, tau, info):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--problems", type=int, default=1200)
+    ap.add_argument("--problems", type=int, default=6000)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--data", default=None, help="reuse a dataset.npz instead of generating")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
@@ -248,7 +248,7 @@ def main():
             valid.append(v)
             ks.append(pr["k"])
             meta.append(pr)
-            if (i + 1) % 50 == 0:
+            if (i + 1) % 250 == 0:
                 log(f"data: {i + 1}/{args.problems} problems ({time.time() - t0:.0f} s)")
         feats, umax, valid, ks = np.array(feats), np.array(umax), np.array(valid), np.array(ks)
         np.savez_compressed(os.path.join(args.out, "dataset.npz"), feats=feats, umax=umax, valid=valid, k=ks)
@@ -264,11 +264,11 @@ def main():
     W = train(oz, Xt, Yt, Mt, rng, steps=args.steps)
     Pr = predict(oz, W, Xv)
     fb = np.array([oz2.Ozaki2.moduli_for_bits(int(ks[val][p_i]), int(b)) for p_i, b in zip(Pv, Tv)])
-    # threshold: smallest tau (on a grid) whose choices meet the target on every held-out row
-    tau_best = 0.995
-    for tau in np.round(np.arange(0.5, 0.996, 0.005), 3):
+    # threshold: smallest tau (on a grid) whose choices meet the target on >= 99.5 % of held-out rows
+    tau_best = 0.999
+    for tau in np.round(np.concatenate([np.arange(0.5, 0.99, 0.01), np.arange(0.99, 0.9991, 0.001)]), 4):
         _, met, _, _ = evaluate(Pr, Yv, Mv, fb, tau)
-        if met.all():
+        if met.mean() >= 0.995:
             tau_best = float(tau)
             break
     s, met, fb_met, opt = evaluate(Pr, Yv, Mv, fb, tau_best)

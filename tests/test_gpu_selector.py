@@ -122,9 +122,10 @@ def test_adam_matches_oracle(oz):
     torch.cuda.synchronize()
     w, m, v = S.adam(W.astype(np.float64), G.astype(np.float64), M.astype(np.float64), V.astype(np.float64), 5, 1e-3,
                      wd=1e-4)
+    # the kernel holds 1 - beta2 in float32 (0.00099998713 for 0.999): up to 1.3e-5 relative on the g^2 term
     assert np.allclose(Wd.cpu().numpy(), w, rtol=1e-6, atol=1e-6)
-    assert np.allclose(Md.cpu().numpy(), m, rtol=1e-5, atol=1e-7)
-    assert np.allclose(Vd.cpu().numpy(), v, rtol=1e-5, atol=1e-9)
+    assert np.allclose(Md.cpu().numpy(), m, rtol=2e-5, atol=1e-7)
+    assert np.allclose(Vd.cpu().numpy(), v, rtol=4e-5, atol=1e-9)
 
 
 def test_training_with_the_kernels_fits_a_rule(oz):
@@ -171,3 +172,57 @@ def test_unloaded_selector_falls_back(oz):
     finally:
         if saved is not None:
             oz.selector_set_weights(saved, tau)
+
+
+GAIN_CASES = [  # dist, m, n, k
+    ("u11", 256, 192, 3000), ("u11", 400, 300, 700), ("wide12", 300, 256, 2000), ("wide12", 128, 512, 5000),
+    ("wide12", 512, 128, 900), ("wide20", 256, 256, 4000), ("ints", 300, 300, 2000), ("ints", 128, 256, 8000),
+    ("phi1", 256, 300, 1500), ("u11", 100, 700, 12000),
+]
+
+
+def _gain_matrix(dist, rows, cols, rng):
+    if dist == "u11":
+        v = rng.uniform(-1, 1, (rows, cols))
+    elif dist.startswith("wide"):
+        e = int(dist[4:])
+        v = (1 + rng.random((rows, cols))) * rng.choice([-1.0, 1.0], (rows, cols)) * np.exp2(
+            rng.integers(-e, e + 1, (rows, cols)).astype(float))
+    elif dist == "ints":
+        v = rng.integers(-1000, 1001, (rows, cols)).astype(float)
+    else:
+        v = rng.uniform(-0.5, 0.5, (rows, cols)) * np.exp(rng.standard_normal((rows, cols)))
+    return np.asfortranarray(v).reshape(-1, order="F")
+
+
+def test_trained_selector_gain(oz):
+    """53-bit target (every entry within 1 ulp of the exact product): the trained network against
+    the method without it (moduli_for_bits(k, 53)), on problems it was not trained on"""
+    from oz2 import check
+    saved, _ = oz.selector_weights()
+    if saved is None:
+        pytest.skip("no trained selector weights are compiled into this build")
+    rng = np.random.default_rng(2025)
+    rows = []
+    for (dist, m, n, k) in GAIN_CASES:
+        A = to_dev(_gain_matrix(dist, m, k, rng))
+        B = to_dev(_gain_matrix(dist, k, n, rng))
+        d = dict(ta="N", tb="N", m=m, n=n, k=k, A=A, lda=m, B=B, ldb=k)
+        s_l = oz.selector_choose([d], bits=53)[0][0]
+        s_f = oz2.Ozaki2.moduli_for_bits(k, 53)
+        hi, lo = check.dd_gemm("N", "N", m, n, k, A, m, B, k)
+        C = torch.empty(m * n, dtype=torch.float64, device="cuda")
+        res = {}
+        for name, sv in (("learned", s_l), ("fallback", s_f)):
+            oz.gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m, sv)
+            res[name] = float(check.ulp_errors(C, hi, lo, m, m).max())
+        rows.append((dist, m, n, k, s_l, res["learned"], s_f, res["fallback"]))
+        log(f"{dist} {m}x{n}x{k}: learned s={s_l} max {res['learned']:.3g} ulp | moduli_for_bits s={s_f} "
+            f"max {res['fallback']:.3g} ulp")
+    met_l = sum(r[5] <= 1.0 for r in rows)
+    met_f = sum(r[7] <= 1.0 for r in rows)
+    log(f"target met: learned {met_l}/{len(rows)}, moduli_for_bits {met_f}/{len(rows)}")
+    assert met_l >= len(rows) - 1 and met_l > met_f
+    for r in rows:
+        if r[0] == "ints":  # exact integer data needs a handful of moduli, not 15-16
+            assert r[4] <= 8 < r[6] and r[5] <= 1.0, r
