@@ -42,7 +42,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--workload", default="dgemm8192", choices=["dgemm8192", "zgemm4096", "wide4000", "ksplit"],
+    ap.add_argument("--workload", default="dgemm8192",
+                    choices=["dgemm8192", "zgemm4096", "wide4000", "ksplit", "batch"],
                     help="dgemm8192 (default line), or one of the other measured configurations")
     ap.add_argument("--batch-count", type=int, default=256)
     return ap.parse_args()
@@ -132,7 +133,10 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
     s_mfb = [oz2.Ozaki2.moduli_for_bits(p["k"], bits) for p in probs]
     t_plain = timed(lambda: plain(s_mfb))
     ulp_plain = max_ulp()
-    b = oz2.Batch(oz, [dict(p, num_moduli=0) for p in probs], bits=bits, graph=True, select=True)
+    descs = [dict(p, num_moduli=0) for p in probs]
+    tune = oz.batch_autotune(descs, bits=bits, select=True)
+    b = oz2.Batch(oz, descs, bits=bits, graph=True, select=True, tiles=tune["tiles"],
+                  persistent=tune["persistent"], largest_first=tune["largest_first"])
     t_batch = timed(b.run)
     ulp_batch = max_ulp()
     s_sel = b.moduli()
@@ -149,7 +153,8 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
             break
     b.close()
     # the same batch with moduli_for_bits per problem: grouping and graph alone
-    b2 = oz2.Batch(oz, [dict(p, num_moduli=sv) for p, sv in zip(probs, s_mfb)], graph=True)
+    b2 = oz2.Batch(oz, [dict(p, num_moduli=sv) for p, sv in zip(probs, s_mfb)], graph=True, tiles=tune["tiles"],
+                   persistent=tune["persistent"], largest_first=tune["largest_first"])
     t_group = timed(b2.run)
     b2.close()
     oz.set_profiling(True)
@@ -163,7 +168,10 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
         "learned_grouped_graph": {"tflops": round(flops / t_batch / 1e12, 3), "ms": round(t_batch * 1e3, 3),
                                   "max_ulp": ulp_batch, "mean_s": sum(s_sel) / len(s_sel),
                                   "s_hist": {str(v): s_sel.count(v) for v in sorted(set(s_sel))},
-                                  "launches_per_run": info["launches"], "workspace_gib": round(info["workspace_bytes"] / 2 ** 30, 2)},
+                                  "launches_per_run": info["launches"], "workspace_gib": round(info["workspace_bytes"] / 2 ** 30, 2),
+                                  "autotuned": {"persistent": tune["persistent"], "largest_first": tune["largest_first"],
+                                                "tiles": {str(t): tune["tiles"].count(t) for t in (64, 128, 256)},
+                                                "tune_ms_no_graph": round(tune["ms"], 3)}},
         "speedup_vs_plain": round(t_plain / t_batch, 3),
     }
     if s_eq is not None:
@@ -309,6 +317,28 @@ def main():
         return
     if args.workload in ("wide4000", "ksplit"):
         print(json.dumps(selector_line(args, torch, args.workload)))
+        return
+    if args.workload == "batch":  # the learned-s grouped batch alone (for profiling)
+        oz = oz2.Ozaki2()
+        probs = make_batch(torch, args.batch_count, seed=5)
+        descs = [dict(p, num_moduli=0) for p in probs]
+        tune = oz.batch_autotune(descs, bits=53, select=True)
+        b = oz2.Batch(oz, descs, bits=53, graph=True, select=True, tiles=tune["tiles"], persistent=tune["persistent"],
+                      largest_first=tune["largest_first"])
+        for _ in range(args.warmup):
+            b.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        flops = sum(2.0 * p["m"] * p["n"] * p["k"] for p in probs)
+        print(json.dumps({"metric": "FP64-equiv TFLOP/s: uneven batch, learned s, grouped + hipGraph",
+                          "value": round(flops / ms / 1e9, 3), "unit": "TFLOP/s", "ms_per_step": round(ms, 3),
+                          "steps": args.steps, "moduli": b.moduli(), "tiles": b.tiles(), "tune": tune, **b.info()}))
+        b.close()
+        oz.close()
         return
     m, n, k, s = args.m, args.n, args.k, args.s
     alpha, beta = 1.5, 0.5

@@ -108,10 +108,19 @@ typedef struct {
     int num_moduli;
 } oz2_gemm_desc_t;
 typedef struct oz2_batch_s* oz2_batch;
-enum { OZ2_BATCH_GRAPH = 1, OZ2_BATCH_SELECT = 2 };
+/* GRAPH: capture the launch sequence in a hipGraph; SELECT: learned num_moduli for entries with 0;
+ * PERSISTENT: persistent GEMM schedule (a fixed grid pulling tiles), else one workgroup per tile;
+ * PROBLEM_ORDER: GEMM work in problem order, else largest first */
+enum { OZ2_BATCH_GRAPH = 1, OZ2_BATCH_SELECT = 2, OZ2_BATCH_PERSISTENT = 4, OZ2_BATCH_PROBLEM_ORDER = 8 };
 int oz2_batch_create(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, const int* tiles, int flags,
                      oz2_batch* out);
 int oz2_batch_run(oz2_batch b);
+/* Autotuner: measures whole runs of candidate plans for these problems (tile policies x GEMM
+ * schedules) and returns the fastest: tiles_out (count ints, for oz2_batch_create), flags_out (the
+ * OZ2_BATCH_PERSISTENT / OZ2_BATCH_PROBLEM_ORDER bits to pass on), best_ms.  Runs the batch about
+ * 40 times (C is overwritten); num_moduli as in oz2_batch_create with these flags (SELECT honoured). */
+int oz2_batch_autotune(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, int flags, int* tiles_out,
+                       int* flags_out, double* best_ms);
 /* chosen moduli / macro tiles per problem (count ints each) */
 int oz2_batch_moduli(oz2_batch b, int* out);
 int oz2_batch_tiles(oz2_batch b, int* out);

@@ -19,8 +19,8 @@ k::Tile tile_of_rows(int t) { return t == 256 ? k::Tile::T256 : t == 128 ? k::Ti
 }  // namespace
 
 BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector<int>& s,
-                     const std::vector<int>& tiles, bool use_graph)
-    : eng_(eng), d_(std::move(descs)) {
+                     const std::vector<int>& tiles, bool use_graph, BatchOptions opt)
+    : eng_(eng), d_(std::move(descs)), opt_(opt) {
     const size_t np = d_.size();
     if (s.size() != np) throw std::invalid_argument("oz2: batch needs one number of moduli per problem");
     if (!tiles.empty() && tiles.size() != np) throw std::invalid_argument("oz2: batch needs one tile per problem");
@@ -89,10 +89,11 @@ BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector
     std::vector<k::GemmJob> gj;
     std::vector<k::CrtArgs> cj;
     std::vector<size_t> by_cost = work;
-    std::stable_sort(by_cost.begin(), by_cost.end(), [&](size_t a, size_t b) {
-        auto cost = [&](size_t i) { return static_cast<double>(pp[i].mpad) * pp[i].npad * pp[i].kp * s_[i]; };
-        return cost(a) > cost(b);
-    });
+    if (opt_.largest_first)
+        std::stable_sort(by_cost.begin(), by_cost.end(), [&](size_t a, size_t b) {
+            auto cost = [&](size_t i) { return static_cast<double>(pp[i].mpad) * pp[i].npad * pp[i].kp * s_[i]; };
+            return cost(a) > cost(b);
+        });
     const TabSet* tabs = eng_.device_tables();
     // the jobs hold device pointers into the workspace, whose size depends on the
     // job tables: build once against a null base to size them, then for real
@@ -186,7 +187,8 @@ BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector
     size_t tables = 0;
     for (auto* v : {&scale_, &split_, &gemm_, &crt_})
         for (const k::GroupedLaunch& g : *v)
-            tables += align_up(g.items.size(), 256) + align_up(g.start.size() * 4, 256);
+            tables += align_up(g.items.size(), 256) + align_up(g.start.size() * 4, 256) +
+                      align_up(g.owner.size() * 4, 256);
     const size_t ncounters = gemm_.size();
     ws_size_ = data_bytes + tables + align_up(std::max<size_t>(1, ncounters) * 4, 256) + 4096;
     if (ws_size_ > eng_.budget())
@@ -210,6 +212,11 @@ BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector
             std::memcpy(host.data() + (toff - data_bytes), g.start.data(), g.start.size() * 4);
             g.d_start = reinterpret_cast<const uint32_t*>(ws_ + toff);
             toff += align_up(g.start.size() * 4, 256);
+            if (!g.owner.empty()) {
+                std::memcpy(host.data() + (toff - data_bytes), g.owner.data(), g.owner.size() * 4);
+                g.d_owner = reinterpret_cast<const uint32_t*>(ws_ + toff);
+                toff += align_up(g.owner.size() * 4, 256);
+            }
         }
     counters_ = reinterpret_cast<uint32_t*>(ws_ + toff);
     if (tables) hip_check(hipMemcpy(ws_ + data_bytes, host.data(), tables, hipMemcpyHostToDevice), "batch table upload");
@@ -247,7 +254,8 @@ BatchPlan::~BatchPlan() {
 int BatchPlan::launches() const {
     int n = static_cast<int>(special_.size());
     for (auto* v : {&scale_, &split_, &crt_}) n += static_cast<int>(v->size());
-    return n + 2 * static_cast<int>(gemm_.size());  // each GEMM launch follows a counter reset
+    // a persistent GEMM launch follows a counter reset
+    return n + (opt_.persistent ? 2 : 1) * static_cast<int>(gemm_.size());
 }
 
 void BatchPlan::enqueue(hipStream_t st) {
@@ -257,7 +265,8 @@ void BatchPlan::enqueue(hipStream_t st) {
     }
     for (const k::GroupedLaunch& g : scale_) k::launch_scale_split_grouped(st, g);
     for (const k::GroupedLaunch& g : split_) k::launch_scale_split_grouped(st, g);
-    for (size_t i = 0; i < gemm_.size(); ++i) k::launch_gemm_grouped(st, gemm_[i], counters_ + i, eng_.compute_units());
+    for (size_t i = 0; i < gemm_.size(); ++i)
+        k::launch_gemm_grouped(st, gemm_[i], counters_ + i, eng_.compute_units(), opt_.persistent);
     for (const k::GroupedLaunch& g : crt_) k::launch_crt_grouped(st, g);
 }
 
@@ -268,8 +277,85 @@ void BatchPlan::run(hipStream_t st) {
 }
 
 void BatchPlan::run_gemm_only(hipStream_t st) {
-    for (size_t i = 0; i < gemm_.size(); ++i) k::launch_gemm_grouped(st, gemm_[i], counters_ + i, eng_.compute_units());
+    for (size_t i = 0; i < gemm_.size(); ++i)
+        k::launch_gemm_grouped(st, gemm_[i], counters_ + i, eng_.compute_units(), opt_.persistent);
     hip_check(hipGetLastError(), "batch gemm launch");
+}
+
+TuneResult autotune_batch(Engine& eng, const std::vector<GemmDesc>& d, const std::vector<int>& s, int reps) {
+    TuneResult best;
+    best.ms = 1e300;
+    hipStream_t st = eng.stream();
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event create");
+    hip_check(hipEventCreate(&e1), "event create");
+    auto timed = [&](const std::vector<int>& tiles, BatchOptions opt) {
+        BatchPlan plan(eng, d, s, tiles, false, opt);
+        plan.run(st);  // warm (and fills the planes once)
+        float bestms = 1e30f;
+        for (int r = 0; r < std::max(1, reps); ++r) {
+            hip_check(hipEventRecord(e0, st), "event record");
+            plan.run(st);
+            hip_check(hipEventRecord(e1, st), "event record");
+            hip_check(hipEventSynchronize(e1), "event synchronize");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+            bestms = std::min(bestms, ms);
+        }
+        return static_cast<double>(bestms);
+    };
+    auto policy = [&](int64_t a, int64_t b) {
+        std::vector<int> t(d.size());
+        for (size_t i = 0; i < d.size(); ++i) {
+            const int64_t x = std::min(d[i].m, d[i].n);
+            t[i] = x >= a ? 256 : x >= b ? 128 : 64;
+        }
+        return t;
+    };
+    const int64_t inf = int64_t{1} << 40;
+    struct Cand {
+        std::string name;
+        std::vector<int> tiles;
+    };
+    std::vector<Cand> cands{{"heuristic", {}}};
+    for (auto [a, b] : {std::pair<int64_t, int64_t>{192, 64}, {256, 96}, {384, 128}, {512, 160}, {768, 256},
+                        {inf, 128}, {inf, 64}})
+        cands.push_back({"tile256>=" + (a >= inf ? std::string("inf") : std::to_string(a)) + ",tile128>=" +
+                             std::to_string(b),
+                         policy(a, b)});
+    try {
+        for (const Cand& c : cands)
+            for (bool pers : {false, true}) {
+                BatchOptions o;
+                o.persistent = pers;
+                const double ms = timed(c.tiles, o);
+                best.trials.push_back({c.name + (pers ? ",persistent" : ",one-tile-per-block"), ms});
+                if (ms < best.ms) {
+                    best.ms = ms;
+                    best.opt = o;
+                    best.tiles = c.tiles;
+                }
+            }
+        BatchOptions o = best.opt;
+        o.largest_first = false;
+        const double ms = timed(best.tiles, o);
+        best.trials.push_back({"winner in problem order", ms});
+        if (ms < best.ms) {
+            best.ms = ms;
+            best.opt = o;
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best.tiles.empty()) {  // the heuristic won: report its choices
+        BatchPlan plan(eng, d, s, {}, false, best.opt);
+        best.tiles = plan.tiles();
+    }
+    return best;
 }
 
 }  // namespace oz2

@@ -14,6 +14,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "engine.hpp"
@@ -34,12 +36,18 @@ struct GemmDesc {
     int s = 0;  // number of moduli; <= 0: chosen by the plan (selector, else moduli_for_bits)
 };
 
+// schedule of the grouped INT8 GEMM launches (chosen by measurement, see autotune_batch)
+struct BatchOptions {
+    bool persistent = false;     // fixed grid pulling tiles from a counter, else one workgroup per tile
+    bool largest_first = true;   // GEMM work listed by decreasing size, else in problem order
+};
+
 class BatchPlan {
 public:
     // s[i] > 0 fixes problem i's moduli; the engine supplies tables, stream and device facts.
     // tiles[i] in {256, 128, 64} forces a macro tile (empty: engine heuristic per problem).
     BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector<int>& s, const std::vector<int>& tiles,
-              bool use_graph);
+              bool use_graph, BatchOptions opt = {});
     ~BatchPlan();
     BatchPlan(const BatchPlan&) = delete;
     BatchPlan& operator=(const BatchPlan&) = delete;
@@ -56,6 +64,7 @@ private:
     void enqueue(hipStream_t st);
     Engine& eng_;
     std::vector<GemmDesc> d_;
+    BatchOptions opt_;
     std::vector<int> s_, tile_rows_;
     std::vector<size_t> special_;  // k == 0 or alpha == 0: C = beta C
     std::vector<k::GroupedLaunch> scale_, split_, gemm_, crt_;
@@ -65,5 +74,18 @@ private:
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
 };
+
+// Tile shapes and schedule for one batch, chosen by measuring whole runs of candidate
+// plans (the union of all problems fills the GPU, so the per-problem heuristic is not
+// the best choice): tile policies tile(min(m, n)) = 256 if >= a, 128 if >= b, else 64,
+// plus the single-problem heuristic, each with and without the persistent schedule;
+// then problem order instead of largest first for the winner.
+struct TuneResult {
+    std::vector<int> tiles;
+    BatchOptions opt;
+    double ms = 0;
+    std::vector<std::pair<std::string, double>> trials;  // candidate -> best ms
+};
+TuneResult autotune_batch(Engine& eng, const std::vector<GemmDesc>& d, const std::vector<int>& s, int reps = 3);
 
 }  // namespace oz2

@@ -287,30 +287,63 @@ int oz2_moduli(int num_moduli, uint16_t* out) {
     });
 }
 
+namespace {
+// descriptors and moduli of a batch as oz2_batch_create chooses them
+void batch_inputs(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, int flags,
+                  std::vector<oz2::GemmDesc>& d, std::vector<int>& s) {
+    if (count < 0 || (count > 0 && !descs)) throw std::invalid_argument("oz2: bad batch");
+    d.assign(count, oz2::GemmDesc{});
+    s.assign(count, 0);
+    for (int i = 0; i < count; ++i) {
+        const oz2_gemm_desc_t& c = descs[i];
+        d[i] = oz2::GemmDesc{to_op(c.ta), to_op(c.tb), c.m, c.n, c.k, c.alpha, c.A, c.lda,
+                             c.B, c.ldb, c.beta, c.C, c.ldc, c.num_moduli};
+        s[i] = c.num_moduli > 0 ? c.num_moduli : oz2::moduli_for_bits(std::max<int64_t>(c.k, 1), bits);
+    }
+    // shapes are validated by the plan; the selector only reads problems that would pass it
+    for (const auto& g : d)
+        if (g.m < 0 || g.n < 0 || g.k < 0) throw std::invalid_argument("oz2: negative dimension");
+    if ((flags & OZ2_BATCH_SELECT) != 0) select_moduli(h, d, bits, s);
+}
+
+oz2::BatchOptions batch_options(int flags) {
+    oz2::BatchOptions o;
+    o.persistent = (flags & OZ2_BATCH_PERSISTENT) != 0;
+    o.largest_first = (flags & OZ2_BATCH_PROBLEM_ORDER) == 0;
+    return o;
+}
+}  // namespace
+
 int oz2_batch_create(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, const int* tiles, int flags,
                      oz2_batch* out) {
     return guard([&] {
         need(h);
         if (!out) throw std::invalid_argument("oz2: null output");
-        if (count < 0 || (count > 0 && !descs)) throw std::invalid_argument("oz2: bad batch");
-        std::vector<oz2::GemmDesc> d(count);
-        std::vector<int> s(count), t;
-        for (int i = 0; i < count; ++i) {
-            const oz2_gemm_desc_t& c = descs[i];
-            d[i] = oz2::GemmDesc{to_op(c.ta), to_op(c.tb), c.m, c.n, c.k, c.alpha, c.A, c.lda,
-                                 c.B, c.ldb, c.beta, c.C, c.ldc, c.num_moduli};
-            s[i] = c.num_moduli > 0 ? c.num_moduli : oz2::moduli_for_bits(std::max<int64_t>(c.k, 1), bits);
-        }
-        // validation of shapes happens in the plan; the selector only reads problems that pass it
-        for (const auto& g : d)
-            if (g.m < 0 || g.n < 0 || g.k < 0) throw std::invalid_argument("oz2: negative dimension");
-        if ((flags & OZ2_BATCH_SELECT) != 0) select_moduli(h, d, bits, s);
+        std::vector<oz2::GemmDesc> d;
+        std::vector<int> s, t;
+        batch_inputs(h, count, descs, bits, flags, d, s);
         if (tiles) t.assign(tiles, tiles + count);
         auto b = std::make_unique<oz2_batch_s>();
         b->h = h;
         b->count = count;
-        b->plan = std::make_unique<oz2::BatchPlan>(h->eng, std::move(d), s, t, (flags & OZ2_BATCH_GRAPH) != 0);
+        b->plan = std::make_unique<oz2::BatchPlan>(h->eng, std::move(d), s, t, (flags & OZ2_BATCH_GRAPH) != 0,
+                                                   batch_options(flags));
         *out = b.release();
+    });
+}
+
+int oz2_batch_autotune(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, int flags, int* tiles_out,
+                       int* flags_out, double* best_ms) {
+    return guard([&] {
+        need(h);
+        std::vector<oz2::GemmDesc> d;
+        std::vector<int> s;
+        batch_inputs(h, count, descs, bits, flags, d, s);
+        const oz2::TuneResult r = oz2::autotune_batch(h->eng, d, s);
+        if (tiles_out) std::copy(r.tiles.begin(), r.tiles.end(), tiles_out);
+        if (flags_out)
+            *flags_out = (r.opt.persistent ? OZ2_BATCH_PERSISTENT : 0) | (r.opt.largest_first ? 0 : OZ2_BATCH_PROBLEM_ORDER);
+        if (best_ms) *best_ms = r.ms;
     });
 }
 

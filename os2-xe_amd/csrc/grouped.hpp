@@ -24,8 +24,10 @@ struct GroupedLaunch {
     size_t item_bytes = 0;
     std::vector<uint8_t> items;    // n packed items
     std::vector<uint32_t> start{0};
+    std::vector<uint32_t> owner;   // optional: item of every block (one load instead of a search)
     const void* d_items = nullptr; // device copies, set by the owner before launching
     const uint32_t* d_start = nullptr;
+    const uint32_t* d_owner = nullptr;
 
     template <typename T>
     void add(const T& item, uint32_t nblocks) {

@@ -261,28 +261,42 @@ __global__ void __launch_bounds__(256, MINB) igemm_residue_kernel(GemmArgs p) {
     gemm_tile<BM, BN>(p, plane, tm, tn, smem);
 }
 
-// Persistent grouped form: a fixed grid takes work items (problem, plane, tile)
-// from a counter until all `total` are done; items are numbered problem by
-// problem in the order the host listed them (largest first balances the tail).
+// Grouped forms over many problems; owner[w] is the item (problem, k-chunk) of
+// work index w and start[] its first index, items listed largest first.
+// Persistent: a fixed grid takes work indices from a counter until all are done.
 template <int BM, int BN, int MINB>
 __global__ void __launch_bounds__(256, MINB) igemm_grouped_kernel(const GemmArgs* __restrict__ items,
-                                                                  const uint32_t* __restrict__ start, uint32_t n,
-                                                                  uint32_t* __restrict__ counter) {
+                                                                  const uint32_t* __restrict__ start,
+                                                                  const uint32_t* __restrict__ owner,
+                                                                  uint32_t total, uint32_t* __restrict__ counter) {
     __shared__ v4i smem[TileShape<BM, BN>::SMEM / 16];
     __shared__ uint32_t next;
-    const uint32_t total = start[n];
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
         __syncthreads();
         const uint32_t w = __builtin_amdgcn_readfirstlane(next);  // uniform: item fields stay in SGPRs
         __syncthreads();  // also: the previous tile's epilogue is done with LDS
         if (w >= total) break;
-        const uint32_t it = owner_of(start, n, w);
+        const uint32_t it = owner[w];
         const GemmArgs& p = items[it];
         uint32_t plane, tm, tn;
         tile_coords(p, w - start[it], plane, tm, tn);
         gemm_tile<BM, BN>(p, plane, tm, tn, smem);
     }
+}
+
+// One workgroup per tile, work indices spread over the XCDs in contiguous ranges.
+template <int BM, int BN, int MINB>
+__global__ void __launch_bounds__(256, MINB) igemm_grouped_blocks_kernel(const GemmArgs* __restrict__ items,
+                                                                         const uint32_t* __restrict__ start,
+                                                                         const uint32_t* __restrict__ owner) {
+    __shared__ v4i smem[TileShape<BM, BN>::SMEM / 16];
+    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t it = owner[w];
+    const GemmArgs& p = items[it];
+    uint32_t plane, tm, tn;
+    tile_coords(p, w - start[it], plane, tm, tn);
+    gemm_tile<BM, BN>(p, plane, tm, tn, smem);
 }
 
 __global__ void reset_counters(uint32_t* c, int n) {
@@ -303,32 +317,40 @@ void prep_gemm(std::vector<GroupedLaunch>& out, const std::vector<GemmJob>& jobs
         if (nblk == 0 || j.args.klen <= 0) continue;
         const int t = static_cast<int>(j.tile);
         by[t].kind = t;
+        by[t].owner.insert(by[t].owner.end(), nblk, by[t].n);
         by[t].add(j.args, nblk);
     }
     for (GroupedLaunch& g : by)
         if (g.n) out.push_back(std::move(g));
 }
 
-void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* counter, int cus) {
+void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* counter, int cus, bool persistent) {
     if (g.n == 0) return;
-    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, counter, 1);
     const Tile t = static_cast<Tile>(g.kind);
-    const uint32_t grid = std::min<uint32_t>(g.blocks, static_cast<uint32_t>(cus * tile_blocks_per_cu(t)));
     const GemmArgs* items = static_cast<const GemmArgs*>(g.d_items);
-    switch (t) {
-        case Tile::T256:
-            hipLaunchKernelGGL((igemm_grouped_kernel<256, 256, 1>), dim3(grid), dim3(256), 0, st, items, g.d_start, g.n,
-                               counter);
-            break;
-        case Tile::T128:
-            hipLaunchKernelGGL((igemm_grouped_kernel<128, 128, 2>), dim3(grid), dim3(256), 0, st, items, g.d_start, g.n,
-                               counter);
-            break;
-        case Tile::T64:
-            hipLaunchKernelGGL((igemm_grouped_kernel<64, 64, 4>), dim3(grid), dim3(256), 0, st, items, g.d_start, g.n,
-                               counter);
-            break;
+    if (persistent) {
+        hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, counter, 1);
+        const uint32_t grid = std::min<uint32_t>(g.blocks, static_cast<uint32_t>(cus * tile_blocks_per_cu(t)));
+#define OZ2_LAUNCH(BM, MINB)                                                                                     \
+    hipLaunchKernelGGL((igemm_grouped_kernel<BM, BM, MINB>), dim3(grid), dim3(256), 0, st, items, g.d_start,    \
+                       g.d_owner, g.blocks, counter)
+        switch (t) {
+            case Tile::T256: OZ2_LAUNCH(256, 1); break;
+            case Tile::T128: OZ2_LAUNCH(128, 2); break;
+            case Tile::T64: OZ2_LAUNCH(64, 4); break;
+        }
+#undef OZ2_LAUNCH
+        return;
     }
+#define OZ2_LAUNCH(BM, MINB)                                                                                     \
+    hipLaunchKernelGGL((igemm_grouped_blocks_kernel<BM, BM, MINB>), dim3(g.blocks), dim3(256), 0, st, items,     \
+                       g.d_start, g.d_owner)
+    switch (t) {
+        case Tile::T256: OZ2_LAUNCH(256, 1); break;
+        case Tile::T128: OZ2_LAUNCH(128, 2); break;
+        case Tile::T64: OZ2_LAUNCH(64, 4); break;
+    }
+#undef OZ2_LAUNCH
 }
 
 void igemm_residue(hipStream_t st, Tile tile, const GemmArgs& a) {

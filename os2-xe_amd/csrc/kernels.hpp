@@ -115,8 +115,9 @@ void prep_split(std::vector<GroupedLaunch>& out, const std::vector<SplitArgs>& j
 void prep_gemm(std::vector<GroupedLaunch>& out, const std::vector<GemmJob>& jobs);
 void prep_crt(std::vector<GroupedLaunch>& out, const std::vector<CrtArgs>& jobs);
 bool launch_scale_split_grouped(hipStream_t st, const GroupedLaunch& g);
-// persistent: min(blocks, cus x blocks per CU) workgroups pull tiles from *counter (reset first)
-void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* counter, int cus);
+// persistent: min(blocks, cus x blocks per CU) workgroups pull tiles from *counter (reset first);
+// else one workgroup per tile (counter unused)
+void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* counter, int cus, bool persistent);
 void launch_crt_grouped(hipStream_t st, const GroupedLaunch& g);
 
 }  // namespace k

@@ -85,16 +85,22 @@ OZ2_HD void scaled_mag(uint64_t bits, int e, uint64_t& hi, uint64_t& lo) {
     }
 }
 
-// Offset-binary u = 2^OFF + x (OFF = 62 narrow, 104 wide; x = +-RNE(|v| 2^e)) as
-// limbs: wide: five 21-bit limbs; narrow (u <= 2^63): five 13-bit limbs (the
-// top one <= 2^11), so that the limb sum of limb_residue stays below 2^23.
+// Limb forms of the offset-binary value u = 2^OFF + x (x = +-RNE(|v| 2^e)):
+//   Wide21   OFF = 104, five 21-bit limbs (tk > 62);
+//   Narrow21 OFF = 62, three 21-bit limbs (the top one <= 2^21);
+//   Narrow13 OFF = 62, five 13-bit limbs (the top one <= 2^11): more limbs, cheaper
+//            reduction (the limb sum stays below 2^23, see limb_residue).
+// Both narrow forms give the same residues; the split picks per memory order
+// (strided rows keep fewer registers live with Narrow21).
+enum class Limbs { Wide21, Narrow21, Narrow13 };
+
 // zero: the element is outside the matrix or its row is non-finite, giving x = 0.
-template <bool WIDE>
+template <Limbs F>
 OZ2_HD void to_limbs(uint64_t bits, int e, bool zero, uint32_t (&a)[5]) {
     uint64_t hi = 0, lo = 0;
     if (!zero) scaled_mag(bits, e, hi, lo);
     const bool neg = (bits >> 63) != 0 && !zero;
-    if (WIDE) {  // mag <= 2^100
+    if (F == Limbs::Wide21) {  // mag <= 2^100
         uint64_t uh, ul;
         if (neg) {
             ul = 0 - lo;
@@ -111,53 +117,63 @@ OZ2_HD void to_limbs(uint64_t bits, int e, bool zero, uint32_t (&a)[5]) {
         a[4] = static_cast<uint32_t>((uh >> 20) & M21);
     } else {  // mag <= 2^62, u <= 2^63
         const uint64_t ul = neg ? (1ull << 62) - lo : (1ull << 62) + lo;
-        constexpr uint64_t M13 = 0x1FFF;
-        a[0] = static_cast<uint32_t>(ul & M13);
-        a[1] = static_cast<uint32_t>((ul >> 13) & M13);
-        a[2] = static_cast<uint32_t>((ul >> 26) & M13);
-        a[3] = static_cast<uint32_t>((ul >> 39) & M13);
-        a[4] = static_cast<uint32_t>(ul >> 52);
+        if (F == Limbs::Narrow21) {
+            a[0] = static_cast<uint32_t>(ul & 0x1FFFFF);
+            a[1] = static_cast<uint32_t>((ul >> 21) & 0x1FFFFF);
+            a[2] = static_cast<uint32_t>(ul >> 42);
+            a[3] = a[4] = 0;
+        } else {
+            constexpr uint64_t M13 = 0x1FFF;
+            a[0] = static_cast<uint32_t>(ul & M13);
+            a[1] = static_cast<uint32_t>((ul >> 13) & M13);
+            a[2] = static_cast<uint32_t>((ul >> 26) & M13);
+            a[3] = static_cast<uint32_t>((ul >> 39) & M13);
+            a[4] = static_cast<uint32_t>(ul >> 52);
+        }
     }
 }
 
 // (x + h) mod m in [0, m) for modulus index t (m <= 255) from the limbs of u = 2^OFF + x.
-// wide: v = a_0 + sum a_j (2^(21 j) mod m) + (h - 2^104 mod m) < 2^32, then mod32.
-// narrow: v = a_0 + sum a_j (2^(13 j) mod m) + (h - 2^62 mod m) < 2^23, accumulated
+// 21-bit forms: v = a_0 + sum a_j (2^(21 j) mod m) + (h - 2^OFF mod m) < 2^32, then mod32.
+// Narrow13: v = a_0 + sum a_j (2^(13 j) mod m) + (h - 2^62 mod m) < 2^23, accumulated
 // doubled (2v < 2^24, constants pre-doubled) so that q = mulhi24(2v, mu) =
 // floor(v mu / 2^31) is the Barrett quotient (floor(v/m) or one less) and
 // r = (2v - q 2m) / 2 lies in [0, 2m).
-template <bool WIDE>
+template <Limbs F>
 OZ2_HD uint32_t limb_residue(const uint32_t (&a)[5], const TabSet* __restrict__ tb, int t) {
-    if (WIDE) {
-        uint32_t v = a[0] + tb->off[1][t];
-        v += mul24(a[1], tb->c21[t][0]);
-        v += mul24(a[2], tb->c21[t][1]);
+    if (F == Limbs::Narrow13) {
+        uint32_t v2 = 2 * a[0] + tb->off2[t];
+        v2 += mul24(a[1], tb->c13x2[t][0]);
+        v2 += mul24(a[2], tb->c13x2[t][1]);
+        v2 += mul24(a[3], tb->c13x2[t][2]);
+        v2 += mul24(a[4], tb->c13x2[t][3]);
+        const uint32_t q = mulhi24(v2, tb->mu[t]);
+        const uint32_t r = (v2 - mul24(q, tb->m2[t])) >> 1;
+        return umin(r, r - tb->m[t]);
+    }
+    constexpr bool W = F == Limbs::Wide21;
+    uint32_t v = a[0] + tb->off[W ? 1 : 0][t];
+    v += mul24(a[1], tb->c21[t][0]);
+    v += mul24(a[2], tb->c21[t][1]);
+    if (W) {
         v += mul24(a[3], tb->c21[t][2]);
         v += mul24(a[4], tb->c21[t][3]);
-        return mod32(v, tb->m[t], tb->mu[t], tb->c16[t]);
     }
-    uint32_t v2 = 2 * a[0] + tb->off2[t];
-    v2 += mul24(a[1], tb->c13x2[t][0]);
-    v2 += mul24(a[2], tb->c13x2[t][1]);
-    v2 += mul24(a[3], tb->c13x2[t][2]);
-    v2 += mul24(a[4], tb->c13x2[t][3]);
-    const uint32_t q = mulhi24(v2, tb->mu[t]);
-    const uint32_t r = (v2 - mul24(q, tb->m2[t])) >> 1;
-    return umin(r, r - tb->m[t]);
+    return mod32(v, tb->m[t], tb->mu[t], tb->c16[t]);
 }
 
 // int8 byte of the symmetric residue of x for modulus index t (m = 256 only at t = 0)
-template <bool WIDE>
+template <Limbs F>
 OZ2_HD uint32_t sym_byte(const uint32_t (&a)[5], const TabSet* __restrict__ tb, int t) {
     if (t == 0) return a[0] & 0xFFu;  // m = 256: x mod 256 = u mod 256
-    return (limb_residue<WIDE>(a, tb, t) - tb->h[t]) & 0xFFu;
+    return (limb_residue<F>(a, tb, t) - tb->h[t]) & 0xFFu;
 }
 
 // symmetric residue value in [-h, m-1-h] (complex split needs the value, not the byte)
-template <bool WIDE>
+template <Limbs F>
 OZ2_HD int32_t sym_value(const uint32_t (&a)[5], const TabSet* __restrict__ tb, int t) {
     if (t == 0) return static_cast<int32_t>((a[0] + 127u) & 0xFFu) - 127;
-    return static_cast<int32_t>(limb_residue<WIDE>(a, tb, t)) - static_cast<int32_t>(tb->h[t]);
+    return static_cast<int32_t>(limb_residue<F>(a, tb, t)) - static_cast<int32_t>(tb->h[t]);
 }
 
 // GEMM epilogue: int32 dot product x (as bits) -> x mod m in [0, m), m <= 255.

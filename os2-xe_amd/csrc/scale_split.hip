@@ -159,6 +159,9 @@ __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, 
     const int32_t e = rowok ? a.ex[r] : 0;
     const bool rowzero = !rowok || e == INT32_MIN;
 
+    // narrow values: 13-bit limbs when rows are contiguous, 21-bit limbs (fewer live
+    // registers for the strided loads) when they are not
+    constexpr Limbs F = WIDE ? Limbs::Wide21 : KCONTIG ? Limbs::Narrow13 : Limbs::Narrow21;
     uint32_t limbs[16][5];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -169,7 +172,7 @@ __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, 
             const double* src = KCONTIG ? a.X + r * a.ldx + p : a.X + r + p * a.ldx;
             bits = static_cast<uint64_t>(__double_as_longlong(*src));
         }
-        to_limbs<WIDE>(bits, e, z, limbs[q]);
+        to_limbs<F>(bits, e, z, limbs[q]);
     }
     int8_t* out = a.P + plane_offset(r, p0, a.kp);
     for (int t = 0; t < a.s; ++t) {
@@ -178,7 +181,7 @@ __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, 
         for (int g = 0; g < 4; ++g) {
             uint32_t acc = 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc |= sym_byte<WIDE>(limbs[4 * g + q], tb, t) << (8 * q);
+            for (int q = 0; q < 4; ++q) acc |= sym_byte<F>(limbs[4 * g + q], tb, t) << (8 * q);
             w[g] = acc;
         }
         *reinterpret_cast<uint4*>(out + t * a.plane) = make_uint4(w[0], w[1], w[2], w[3]);
@@ -206,6 +209,7 @@ __device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, 
     const int32_t e = rowok ? a.ex[r] : 0;
     const bool rowzero = !rowok || e == INT32_MIN;
 
+    constexpr Limbs F = WIDE ? Limbs::Wide21 : KCONTIG ? Limbs::Narrow13 : Limbs::Narrow21;
     uint32_t lr[4][5], li[4][5];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -217,8 +221,8 @@ __device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, 
             br = static_cast<uint64_t>(__double_as_longlong(src[0]));
             bi = static_cast<uint64_t>(__double_as_longlong(src[1]));
         }
-        to_limbs<WIDE>(br, e, z, lr[q]);
-        to_limbs<WIDE>(bi, e, z, li[q]);
+        to_limbs<F>(br, e, z, lr[q]);
+        to_limbs<F>(bi, e, z, li[q]);
     }
     int8_t* out = a.P + plane_offset(r, p0, a.kp);
     for (int t = 0; t < a.s; ++t) {
@@ -227,8 +231,8 @@ __device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, 
         uint32_t w0 = 0, w1 = 0, w2 = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int32_t s0 = sym_value<WIDE>(lr[q], tb, t);
-            int32_t s1 = sym_value<WIDE>(li[q], tb, t);
+            const int32_t s0 = sym_value<F>(lr[q], tb, t);
+            int32_t s1 = sym_value<F>(li[q], tb, t);
             if (CONJ) s1 = -s1;
             int32_t ss = s0 + s1;
             ss = ss > m - 1 - h ? ss - m : ss;

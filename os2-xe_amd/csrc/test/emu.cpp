@@ -17,31 +17,23 @@ const TabSet* tables() {
     }();
     return t.data();
 }
-}  // namespace
-
-extern "C" {
-
-// symmetric int8 residues of RNE(v_i * 2^e_i): planes[t * n + i]; e = INT32_MIN gives 0
-void emu_split(const double* v, const int32_t* e, int64_t n, int s, int tk, int8_t* planes) {
+template <math::Limbs F>
+void split_form(const double* v, const int32_t* e, int64_t n, int s, int8_t* planes) {
     const TabSet* tb = tables() + s;
-    const bool wide = tk > kNarrowScaleBits;
     for (int64_t i = 0; i < n; ++i) {
         uint32_t a[5];
         uint64_t bits;
         __builtin_memcpy(&bits, v + i, 8);
         const bool zero = e[i] == INT32_MIN;
-        if (wide) math::to_limbs<true>(bits, zero ? 0 : e[i], zero, a);
-        else math::to_limbs<false>(bits, zero ? 0 : e[i], zero, a);
-        for (int t = 0; t < s; ++t)
-            planes[t * n + i] = static_cast<int8_t>(wide ? math::sym_byte<true>(a, tb, t) : math::sym_byte<false>(a, tb, t));
+        math::to_limbs<F>(bits, zero ? 0 : e[i], zero, a);
+        for (int t = 0; t < s; ++t) planes[t * n + i] = static_cast<int8_t>(math::sym_byte<F>(a, tb, t));
     }
 }
 
-// complex: groups Re, Im (negated if conj), Re+Im reduced to [-h, m-1-h]: planes[(g*s + t)*n + i]
-void emu_split_cplx(const double* re, const double* im, const int32_t* e, int64_t n, int s, int tk, int conj,
-                    int8_t* planes) {
+template <math::Limbs F>
+void split_cplx_form(const double* re, const double* im, const int32_t* e, int64_t n, int s, int conj,
+                     int8_t* planes) {
     const TabSet* tb = tables() + s;
-    const bool wide = tk > kNarrowScaleBits;
     for (int64_t i = 0; i < n; ++i) {
         uint32_t ar[5], ai[5];
         uint64_t br, bi;
@@ -49,12 +41,12 @@ void emu_split_cplx(const double* re, const double* im, const int32_t* e, int64_
         __builtin_memcpy(&bi, im + i, 8);
         const bool zero = e[i] == INT32_MIN;
         const int ee = zero ? 0 : e[i];
-        if (wide) { math::to_limbs<true>(br, ee, zero, ar); math::to_limbs<true>(bi, ee, zero, ai); }
-        else { math::to_limbs<false>(br, ee, zero, ar); math::to_limbs<false>(bi, ee, zero, ai); }
+        math::to_limbs<F>(br, ee, zero, ar);
+        math::to_limbs<F>(bi, ee, zero, ai);
         for (int t = 0; t < s; ++t) {
             const int32_t m = static_cast<int32_t>(tb->m[t]), h = static_cast<int32_t>(tb->h[t]);
-            const int32_t s0 = wide ? math::sym_value<true>(ar, tb, t) : math::sym_value<false>(ar, tb, t);
-            int32_t s1 = wide ? math::sym_value<true>(ai, tb, t) : math::sym_value<false>(ai, tb, t);
+            const int32_t s0 = math::sym_value<F>(ar, tb, t);
+            int32_t s1 = math::sym_value<F>(ai, tb, t);
             if (conj) s1 = -s1;
             int32_t ss = s0 + s1;
             ss = ss > m - 1 - h ? ss - m : ss;
@@ -64,6 +56,26 @@ void emu_split_cplx(const double* re, const double* im, const int32_t* e, int64_
             planes[(2 * s + t) * n + i] = static_cast<int8_t>(ss);
         }
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+// symmetric int8 residues of RNE(v_i * 2^e_i): planes[t * n + i]; e = INT32_MIN gives 0.
+// narrow13: for tk <= 62, the 13-bit limb form (else the 21-bit one); ignored when wide
+void emu_split(const double* v, const int32_t* e, int64_t n, int s, int tk, int narrow13, int8_t* planes) {
+    if (tk > kNarrowScaleBits) split_form<math::Limbs::Wide21>(v, e, n, s, planes);
+    else if (narrow13) split_form<math::Limbs::Narrow13>(v, e, n, s, planes);
+    else split_form<math::Limbs::Narrow21>(v, e, n, s, planes);
+}
+
+// complex: groups Re, Im (negated if conj), Re+Im reduced to [-h, m-1-h]: planes[(g*s + t)*n + i]
+void emu_split_cplx(const double* re, const double* im, const int32_t* e, int64_t n, int s, int tk, int conj,
+                    int narrow13, int8_t* planes) {
+    if (tk > kNarrowScaleBits) split_cplx_form<math::Limbs::Wide21>(re, im, e, n, s, conj, planes);
+    else if (narrow13) split_cplx_form<math::Limbs::Narrow13>(re, im, e, n, s, conj, planes);
+    else split_cplx_form<math::Limbs::Narrow21>(re, im, e, n, s, conj, planes);
 }
 
 // GEMM epilogue: residues of int32 sums, out[t * n + i] in [0, m_t)

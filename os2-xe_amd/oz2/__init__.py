@@ -115,7 +115,8 @@ _lib.oz2_batch_moduli.argtypes = [_vp, _P(_int)]
 _lib.oz2_batch_tiles.argtypes = [_vp, _P(_int)]
 _lib.oz2_batch_info.argtypes = [_vp, _P(_sz), _P(_int), _P(_int)]
 _lib.oz2_batch_destroy.argtypes = [_vp]
-BATCH_GRAPH, BATCH_SELECT = 1, 2
+_lib.oz2_batch_autotune.argtypes = [_vp, _int, _P(GemmDesc), _int, _int, _P(_int), _P(_int), _P(_dbl)]
+BATCH_GRAPH, BATCH_SELECT, BATCH_PERSISTENT, BATCH_PROBLEM_ORDER = 1, 2, 4, 8
 SEL_FEATURES, SEL_OUTPUTS, SEL_PARAMS = 48, 23, 8791
 _flt = ctypes.c_float
 _lib.oz2_selector_features.argtypes = [_vp, _int, _P(GemmDesc), _int, _P(_flt)]
@@ -134,7 +135,7 @@ EXPORTS = [
     "oz2_release_workspace", "oz2_reserve_workspace", "oz2_workspace_query", "oz2_stage_times", "oz2_last_tiling",
     "oz2_set_tuning", "oz2_get_tuning", "oz2_set_profiling", "oz2_profile_begin", "oz2_profile_end", "oz2_moduli",
     "oz2_unit_split", "oz2_unit_crt", "oz2_batch_create", "oz2_batch_run", "oz2_batch_moduli", "oz2_batch_tiles",
-    "oz2_batch_info", "oz2_batch_destroy", "oz2_selector_features", "oz2_selector_choose",
+    "oz2_batch_info", "oz2_batch_destroy", "oz2_batch_autotune", "oz2_selector_features", "oz2_selector_choose",
     "oz2_selector_set_weights", "oz2_selector_get_weights", "oz2_mlp_forward", "oz2_mlp_backward", "oz2_mlp_adam",
     "oz2_last_error", "oz2_version",
 ]
@@ -197,7 +198,8 @@ class Batch:
     B, ldb, beta, C, ldc and optional num_moduli (<= 0 or absent: chosen by the plan for `bits`).
     The plan keeps the tensors it was given alive; run() recomputes from their current contents."""
 
-    def __init__(self, oz, problems, bits=53, tiles=None, graph=True, select=False):
+    def __init__(self, oz, problems, bits=53, tiles=None, graph=True, select=False, persistent=False,
+                 largest_first=True):
         self._oz = oz
         self._keep = [(p["A"], p["B"], p["C"]) for p in problems]
         n = len(problems)
@@ -205,7 +207,8 @@ class Batch:
         t = None
         if tiles is not None:
             t = (_int * max(1, n))(*tiles)
-        flags = (BATCH_GRAPH if graph else 0) | (BATCH_SELECT if select else 0)
+        flags = ((BATCH_GRAPH if graph else 0) | (BATCH_SELECT if select else 0) |
+                 (BATCH_PERSISTENT if persistent else 0) | (0 if largest_first else BATCH_PROBLEM_ORDER))
         self._b = _vp()
         _check(_lib.oz2_batch_create(oz.handle, n, arr, int(bits), t, flags, ctypes.byref(self._b)))
         self.count = n
@@ -412,3 +415,15 @@ class Ozaki2:
     def mlp_adam(self, W, G, M, V, step, lr, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0):
         _check(_lib.oz2_mlp_adam(self._h, _ptr(W), _ptr(G), _ptr(M), _ptr(V), int(step), float(lr), float(beta1),
                                  float(beta2), float(eps), float(wd)))
+
+    def batch_autotune(self, problems, bits=53, select=False):
+        """-> dict(tiles, persistent, largest_first, ms): the fastest plan found by measuring candidates
+        (overwrites the problems' C); pass the results on to Batch(...)"""
+        n = len(problems)
+        tiles = (_int * max(1, n))()
+        fl = _int(0)
+        ms = _dbl(0)
+        _check(_lib.oz2_batch_autotune(self._h, n, _descs(problems), int(bits), BATCH_SELECT if select else 0, tiles,
+                                       ctypes.byref(fl), ctypes.byref(ms)))
+        return {"tiles": list(tiles)[:n], "persistent": bool(fl.value & BATCH_PERSISTENT),
+                "largest_first": not (fl.value & BATCH_PROBLEM_ORDER), "ms": ms.value}

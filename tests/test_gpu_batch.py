@@ -158,3 +158,35 @@ def test_batch_argument_errors(oz):
     d[0].update(k=140000, ldb=140000, num_moduli=2)  # s too small for k: make_plan's error (hpp:186), before any launch
     with pytest.raises(oz2.OZ2InvalidArgument):
         oz2.Batch(oz, d)
+
+
+@pytest.mark.parametrize("persistent,largest_first", [(True, True), (False, False), (True, False)])
+def test_schedules_identical(oz, persistent, largest_first):
+    log(f"batch schedule persistent={persistent} largest_first={largest_first}")
+    ps = build(PROBLEMS, seed=13)
+    ref = [reference(oz, p) for p in ps]
+    Cs = [to_dev(p["C0"].copy()) for p in ps]
+    b = oz2.Batch(oz, [desc(p, C) for p, C in zip(ps, Cs)], graph=True, persistent=persistent,
+                  largest_first=largest_first)
+    b.run()
+    torch.cuda.synchronize()
+    check_same(ref, Cs, f"persistent={persistent} largest_first={largest_first}")
+    b.close()
+
+
+def test_autotuned_plan(oz):
+    ps = build(PROBLEMS[:8], seed=17)
+    ref = [reference(oz, p) for p in ps]
+    Cs = [to_dev(p["C0"].copy()) for p in ps]
+    d = [desc(p, C) for p, C in zip(ps, Cs)]
+    tune = oz.batch_autotune(d, bits=53)
+    log(f"autotune: {tune}")
+    assert all(t in (0, 64, 128, 256) for t in tune["tiles"]) and tune["ms"] > 0
+    for p, C in zip(ps, Cs):
+        C.copy_(to_dev(p["C0"]))
+    b = oz2.Batch(oz, d, graph=True, tiles=tune["tiles"], persistent=tune["persistent"],
+                  largest_first=tune["largest_first"])
+    b.run()
+    torch.cuda.synchronize()
+    check_same(ref, Cs, "autotuned")
+    b.close()
