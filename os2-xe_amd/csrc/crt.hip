@@ -48,39 +48,62 @@ __device__ __forceinline__ void crt_body(const CrtArgs& a, uint32_t bx) {
         re[e].init();
         if (CPLX) im[e].init();
     }
-    for (int t = 0; t < s; ++t) {
-        const uint32_t m = tb->m[t];
+    // moduli in pairs (2 residues per dot2 step of the limb sums); r = 0 past s
+    for (int t = 0; t < s; t += 2) {
+        const bool two = t + 1 < s;
+        const uint32_t m0 = tb->m[t], m1 = two ? tb->m[t + 1] : 1u;
         if (!CPLX) {
-            uint32_t rs[4] = {0, 0, 0, 0};
-            for (int c = 0; c < a.kch; ++c) {
-                const uint32_t w = *reinterpret_cast<const uint32_t*>(base + c * a.chunk_r + t * a.plane_r);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) rs[e] += (w >> (8 * e)) & 0xFFu;
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) re[e].add(tb, t, a.kch > 1 ? chunk_sum(rs[e], m) : rs[e]);
-        } else {
-            uint32_t p0[4] = {0, 0, 0, 0}, p1[4] = {0, 0, 0, 0}, p2[4] = {0, 0, 0, 0};
-            const int64_t gs = static_cast<int64_t>(s) * a.plane_r;
+            uint32_t r0[4] = {0, 0, 0, 0}, r1[4] = {0, 0, 0, 0};
             for (int c = 0; c < a.kch; ++c) {
                 const uint8_t* b = base + c * a.chunk_r + t * a.plane_r;
                 const uint32_t w0 = *reinterpret_cast<const uint32_t*>(b);
-                const uint32_t w1 = *reinterpret_cast<const uint32_t*>(b + gs);
-                const uint32_t w2 = *reinterpret_cast<const uint32_t*>(b + 2 * gs);
+                const uint32_t w1 = two ? *reinterpret_cast<const uint32_t*>(b + a.plane_r) : 0u;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    p0[e] += (w0 >> (8 * e)) & 0xFFu;
-                    p1[e] += (w1 >> (8 * e)) & 0xFFu;
-                    p2[e] += (w2 >> (8 * e)) & 0xFFu;
+                    r0[e] += (w0 >> (8 * e)) & 0xFFu;
+                    r1[e] += (w1 >> (8 * e)) & 0xFFu;
                 }
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const uint32_t x0 = a.kch > 1 ? chunk_sum(p0[e], m) : p0[e];
-                const uint32_t x1 = a.kch > 1 ? chunk_sum(p1[e], m) : p1[e];
-                const uint32_t x2 = a.kch > 1 ? chunk_sum(p2[e], m) : p2[e];
-                re[e].add(tb, t, chunk_sum(x0 + m - x1, m));               // Re = P0 - P1
-                im[e].add(tb, t, chunk_sum(x2 + 2 * m - x0 - x1, m));      // Im = P2 - P0 - P1
+                const uint32_t x0 = a.kch > 1 ? chunk_sum(r0[e], m0) : r0[e];
+                const uint32_t x1 = a.kch > 1 ? chunk_sum(r1[e], m1) : r1[e];
+                re[e].add2(tb, t >> 1, x0, x1);
+            }
+        } else {
+            uint32_t p0[2][4] = {}, p1[2][4] = {}, p2[2][4] = {};
+            const int64_t gs = static_cast<int64_t>(s) * a.plane_r;
+            for (int c = 0; c < a.kch; ++c) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (h == 1 && !two) break;
+                    const uint8_t* b = base + c * a.chunk_r + (t + h) * a.plane_r;
+                    const uint32_t w0 = *reinterpret_cast<const uint32_t*>(b);
+                    const uint32_t w1 = *reinterpret_cast<const uint32_t*>(b + gs);
+                    const uint32_t w2 = *reinterpret_cast<const uint32_t*>(b + 2 * gs);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        p0[h][e] += (w0 >> (8 * e)) & 0xFFu;
+                        p1[h][e] += (w1 >> (8 * e)) & 0xFFu;
+                        p2[h][e] += (w2 >> (8 * e)) & 0xFFu;
+                    }
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                uint32_t xr[2] = {0, 0}, xi[2] = {0, 0};
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (h == 1 && !two) break;
+                    const uint32_t m = h ? m1 : m0;
+                    const uint32_t x0 = a.kch > 1 ? chunk_sum(p0[h][e], m) : p0[h][e];
+                    const uint32_t x1 = a.kch > 1 ? chunk_sum(p1[h][e], m) : p1[h][e];
+                    const uint32_t x2 = a.kch > 1 ? chunk_sum(p2[h][e], m) : p2[h][e];
+                    xr[h] = chunk_sum(x0 + m - x1, m);           // Re = P0 - P1
+                    xi[h] = chunk_sum(x2 + 2 * m - x0 - x1, m);  // Im = P2 - P0 - P1
+                }
+                re[e].add2(tb, t >> 1, xr[0], xr[1]);
+                im[e].add2(tb, t >> 1, xi[0], xi[1]);
             }
         }
     }

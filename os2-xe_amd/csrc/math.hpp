@@ -27,6 +27,16 @@ OZ2_HD uint32_t mulhi24(uint32_t a, uint32_t b) {
 }
 OZ2_HD uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
+// v_dot2_u32_u16: c + a.lo * b.lo + a.hi * b.hi (16-bit halves, no clamping)
+OZ2_HD uint32_t udot2(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+    return __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b), c, false);
+#else
+    return c + (a & 0xFFFFu) * (b & 0xFFFFu) + (a >> 16) * (b >> 16);
+#endif
+}
+
 // x mod m for x < 2^24, 2 <= m <= 255, mu = floor(2^31 / m) (< 2^24 since m >= 129).
 // Barrett with q = floor(x mu / 2^31): q is floor(x/m) or one less, so r = x - q m
 // lies in [0, 2m) and one conditional subtraction finishes.
@@ -211,6 +221,13 @@ struct CrtAcc {
     OZ2_HD void add(const TabSet* __restrict__ tb, int t, uint32_t r) {
         u = __builtin_fmaf(static_cast<float>(r), tb->frac[t], u);
         for (int l = 0; l < L; ++l) acc[l] += mul24(r, tb->W16[t][l]);
+    }
+    // moduli 2i and 2i+1 at once (r1 = 0 past s): one dot2 per limb with the paired table
+    OZ2_HD void add2(const TabSet* __restrict__ tb, int i, uint32_t r0, uint32_t r1) {
+        u = __builtin_fmaf(static_cast<float>(r0), tb->frac[2 * i], u);
+        u = __builtin_fmaf(static_cast<float>(r1), tb->frac[2 * i + 1], u);
+        const uint32_t rr = r0 | (r1 << 16);
+        for (int l = 0; l < L; ++l) acc[l] = udot2(rr, tb->W16p[i][l], acc[l]);
     }
     // RNE(C') * 2^-shift with C' = sum r_t W_t - q M, q = rint(u)
     OZ2_HD double finish(const TabSet* __restrict__ tb, int shift) const {

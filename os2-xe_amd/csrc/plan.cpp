@@ -171,6 +171,9 @@ void make_tabsets(TabSet* out) {
             for (int l = 0; l < tb.L16; ++l) tb.W16[t][l] = (p.W[t].word32(l / 2) >> (16 * (l % 2))) & 0xFFFFu;
         }
         for (int l = 0; l < tb.L16; ++l) tb.M16[l] = (p.M.word32(l / 2) >> (16 * (l % 2))) & 0xFFFFu;
+        for (int t = 0; t < s; t += 2)
+            for (int l = 0; l < tb.L16; ++l)
+                tb.W16p[t / 2][l] = tb.W16[t][l] | (t + 1 < s ? tb.W16[t + 1][l] << 16 : 0u);
     }
 }
 

@@ -84,6 +84,7 @@ struct TabSet {
     // CRT (hpp:222-236)
     float frac[kMaxModuli];                 // fp32(y_t / m_t)
     uint32_t W16[kMaxModuli][2 * kMaxLimbs];  // 16-bit limbs of W_t
+    uint32_t W16p[kMaxModuli / 2][2 * kMaxLimbs];  // W16[2i][l] | W16[2i+1][l] << 16 (0 past s)
     uint32_t M16[2 * kMaxLimbs];            // 16-bit limbs of M
 };
 // tables for s = 0..24 (entries 0 and 1 unused); y_t and W_t do not depend on k

@@ -96,7 +96,8 @@ void emu_crt(const uint8_t* res, const int32_t* shift, int64_t n, int s, double*
     case NL: {                                             \
         math::CrtAcc<NL> a;                                \
         a.init();                                          \
-        for (int t = 0; t < s; ++t) a.add(tb, t, r[t]);    \
+        for (int t = 0; t < s; t += 2)                     \
+            a.add2(tb, t / 2, r[t], t + 1 < s ? r[t + 1] : 0u); \
         out[i] = a.finish(tb, shift[i]);                   \
         break;                                             \
     }
