@@ -29,6 +29,11 @@ INT8_PEAK_TOPS = 5000.0  # MI355X dense INT8 MFMA, 5.0 POP/s
 HBM_PEAK_GBS = 8000.0
 
 
+# HBM bytes per INT8 GEMM launch measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE passes,
+# each scaled by its reading for a 1 GiB copy kernel): profiles/dgemm8192_s16_hbm_traffic.txt
+MEASURED_GEMM_TRAFFIC = {(8192, 8192, 8192, 16): 14.09e9}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -380,7 +385,8 @@ def main():
         "config": {"workload": f"DGEMM {m}x{n}x{k} op N/N s={s} alpha=1.5 beta=0.5, A B C0 U[-1,1)", "m": m, "n": n,
                    "k": k, "s": s, "tk": oz2.Ozaki2.precision_bits(k, s), "tiling": tiling},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
+                     "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": MEASURED_GEMM_TRAFFIC.get((m, n, k, s)),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "kernel": "igemm_residue_kernel (2*m*n*k*s int8 ops per launch)",
                      "gemm_ms": round(gemm_ms, 4)},
         "stage_ms": {kk: round(v, 4) for kk, v in stage.items()},
