@@ -162,9 +162,10 @@ int oz2_mlp_adam(oz2_handle h, float* W, const float* G, float* M, float* V, int
  * detail::launch_crt as its unit tests call them (tests/test_ozaki2.cpp:215-221, 306-313).
  * oz2_unit_split: rows of op(X) (row r, column p at X[r*ldx + p] if kcontig, else X[r + p*ldx];
  * complex: interleaved, 3 plane groups Re, +-Im, Re+-Im) -> int8 planes [g][t], each rows_pad x kp
- * bytes blocked as the GEMM reads them: element (r, p) at ((r/16)*(kp/64) + p/64)*1024 + (r%16)*64
- * + ((((p/16)%4) ^ ((r/4)%4))*16) + p%16; exponents ex[r]; kp a multiple of 128 and >= k, rows_pad a
- * multiple of 32 and >= rows.
+ * bytes blocked as the GEMM reads them: with q = p%64, element (r, p) at
+ * ((r/16)*(kp/64) + p/64)*1024 + (r%16)*64 + ((((q/2)%4) ^ ((r/4)%4))*16) + 2*(q/8) + q%2
+ * (chunk c of a row holds the k pairs (2c, 2c+1) of each group of 8); exponents ex[r]; kp a
+ * multiple of 128 and >= k, rows_pad a multiple of 32 and >= rows.
  * oz2_unit_crt: residue bytes R[t][j][ldr] (ldr a multiple of 4, >= mb) -> C[i + j*ldc] =
  * alpha*CRT(...)*2^-(D[i]+E[j]) (+ beta*C).  Both run on `stream` and do not synchronize. */
 int oz2_unit_split(void* stream, const double* X, int64_t ldx, int64_t rows, int64_t k, int kcontig, int cplx, int conj,

@@ -130,6 +130,31 @@ void signal(hipStream_t st, void* done) {
     if (done && hipEventRecord(static_cast<hipEvent_t>(done), st) != hipSuccess)
         throw std::runtime_error("oz2: hipEventRecord failed");
 }
+
+// descriptors and moduli of a batch as oz2_batch_create chooses them
+void batch_inputs(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, int flags,
+                  std::vector<oz2::GemmDesc>& d, std::vector<int>& s) {
+    if (count < 0 || (count > 0 && !descs)) throw std::invalid_argument("oz2: bad batch");
+    d.assign(count, oz2::GemmDesc{});
+    s.assign(count, 0);
+    for (int i = 0; i < count; ++i) {
+        const oz2_gemm_desc_t& c = descs[i];
+        d[i] = oz2::GemmDesc{to_op(c.ta), to_op(c.tb), c.m, c.n, c.k, c.alpha, c.A, c.lda,
+                             c.B, c.ldb, c.beta, c.C, c.ldc, c.num_moduli};
+        s[i] = c.num_moduli > 0 ? c.num_moduli : oz2::moduli_for_bits(std::max<int64_t>(c.k, 1), bits);
+    }
+    // shapes are validated by the plan; the selector only reads problems that would pass it
+    for (const auto& g : d)
+        if (g.m < 0 || g.n < 0 || g.k < 0) throw std::invalid_argument("oz2: negative dimension");
+    if ((flags & OZ2_BATCH_SELECT) != 0) select_moduli(h, d, bits, s);
+}
+
+oz2::BatchOptions batch_options(int flags) {
+    oz2::BatchOptions o;
+    o.persistent = (flags & OZ2_BATCH_PERSISTENT) != 0;
+    o.largest_first = (flags & OZ2_BATCH_PROBLEM_ORDER) == 0;
+    return o;
+}
 }  // namespace
 
 extern "C" {
@@ -287,32 +312,6 @@ int oz2_moduli(int num_moduli, uint16_t* out) {
     });
 }
 
-namespace {
-// descriptors and moduli of a batch as oz2_batch_create chooses them
-void batch_inputs(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, int flags,
-                  std::vector<oz2::GemmDesc>& d, std::vector<int>& s) {
-    if (count < 0 || (count > 0 && !descs)) throw std::invalid_argument("oz2: bad batch");
-    d.assign(count, oz2::GemmDesc{});
-    s.assign(count, 0);
-    for (int i = 0; i < count; ++i) {
-        const oz2_gemm_desc_t& c = descs[i];
-        d[i] = oz2::GemmDesc{to_op(c.ta), to_op(c.tb), c.m, c.n, c.k, c.alpha, c.A, c.lda,
-                             c.B, c.ldb, c.beta, c.C, c.ldc, c.num_moduli};
-        s[i] = c.num_moduli > 0 ? c.num_moduli : oz2::moduli_for_bits(std::max<int64_t>(c.k, 1), bits);
-    }
-    // shapes are validated by the plan; the selector only reads problems that would pass it
-    for (const auto& g : d)
-        if (g.m < 0 || g.n < 0 || g.k < 0) throw std::invalid_argument("oz2: negative dimension");
-    if ((flags & OZ2_BATCH_SELECT) != 0) select_moduli(h, d, bits, s);
-}
-
-oz2::BatchOptions batch_options(int flags) {
-    oz2::BatchOptions o;
-    o.persistent = (flags & OZ2_BATCH_PERSISTENT) != 0;
-    o.largest_first = (flags & OZ2_BATCH_PROBLEM_ORDER) == 0;
-    return o;
-}
-}  // namespace
 
 int oz2_batch_create(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, const int* tiles, int flags,
                      oz2_batch* out) {

@@ -19,13 +19,21 @@ namespace oz2 {
 namespace k {
 
 // Byte offset of element (row r, k p) inside one operand plane with kp columns:
-// 16-row bands of kp/64 consecutive 1 KB blocks; inside a block, 64 bytes per
-// row and the four 16-byte k-chunks of row r permuted by XOR with (r >> 2) & 3
-// (the GEMM's DMA copies blocks verbatim to LDS, where that permutation makes
-// the MFMA operand reads free of bank conflicts).
+// 16-row bands of kp/64 consecutive 1 KB blocks; inside a block, 64 bytes per row
+// as four 16-byte chunks.  Byte b of chunk c holds k = block_k(c, b) of the block:
+// chunk c takes the pair (2c, 2c+1) of each group of 8 consecutive k, so the four
+// threads that fill the chunks of one row read 64 contiguous bytes of a
+// k-contiguous row per load.  The GEMM's dot products do not depend on the k order
+// inside a block, which both operands share.  The chunks of row r are permuted by
+// XOR with (r >> 2) & 3: the GEMM's DMA copies blocks verbatim to LDS, where that
+// permutation makes the MFMA operand reads free of bank conflicts.
+__host__ __device__ constexpr int block_k(int c, int b) { return 8 * (b >> 1) + 2 * c + (b & 1); }
+__host__ __device__ inline int64_t chunk_offset(int64_t r, int64_t blk, int c, int64_t kp) {
+    return (((r >> 4) * (kp >> 6) + blk) << 10) + ((r & 15) << 6) + ((c ^ static_cast<int>((r >> 2) & 3)) << 4);
+}
 __host__ __device__ inline int64_t plane_offset(int64_t r, int64_t p, int64_t kp) {
-    return (((r >> 4) * (kp >> 6) + (p >> 6)) << 10) + ((r & 15) << 6) +
-           (((((p >> 4) & 3) ^ ((r >> 2) & 3))) << 4) + (p & 15);
+    const int q = static_cast<int>(p & 63);
+    return chunk_offset(r, p >> 6, (q >> 1) & 3, kp) + (((q >> 3) << 1) | (q & 1));
 }
 
 // ---- scaling -----------------------------------------------------------------

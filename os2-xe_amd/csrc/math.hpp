@@ -18,9 +18,32 @@
 namespace oz2 {
 namespace math {
 
+// The functions below take the constant tables as a pointer of any type with
+// TabSet's members.  Kernels pass const_tab(p): loads through an address-space-4
+// (constant) pointer compile to scalar loads even where the kernel also stores
+// through pointers that could alias the tables (int8 planes, byte residues).
+#if defined(__HIPCC__)
+using ConstTab = const __attribute__((address_space(4))) TabSet*;
+__device__ __forceinline__ ConstTab const_tab(const TabSet* p) { return (ConstTab)p; }
+#endif
+
 // v_mul_u32_u24 / v_mul_hi_u32_u24: low 32 and high 16 bits of a 24 x 24-bit product
 OZ2_HD uint32_t mul24(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
     return static_cast<uint32_t>(static_cast<uint64_t>(a & 0xFFFFFFu) * static_cast<uint64_t>(b & 0xFFFFFFu));
+#endif
+}
+// v_mad_i32_i24: a * b + c with a, b sign-extended from 24 bits (low 32 bits of the result)
+OZ2_HD uint32_t mad_i24(int32_t a, int32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return static_cast<uint32_t>(__mul24(a, b)) + c;
+#else
+    const int64_t sa = static_cast<int64_t>(static_cast<uint64_t>(a) << 40) >> 40;
+    const int64_t sb = static_cast<int64_t>(static_cast<uint64_t>(b) << 40) >> 40;
+    return static_cast<uint32_t>(static_cast<uint64_t>(sa * sb)) + c;
+#endif
 }
 OZ2_HD uint32_t mulhi24(uint32_t a, uint32_t b) {
     return static_cast<uint32_t>((static_cast<uint64_t>(a & 0xFFFFFFu) * static_cast<uint64_t>(b & 0xFFFFFFu)) >> 32);
@@ -37,13 +60,23 @@ OZ2_HD uint32_t udot2(uint32_t a, uint32_t b, uint32_t c) {
 #endif
 }
 
+// v_dot4_u32_u8: c + the sum of the four byte products (no clamping)
+OZ2_HD uint32_t udot4(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_udot4(a, b, c, false);
+#else
+    for (int i = 0; i < 32; i += 8) c += ((a >> i) & 0xFFu) * ((b >> i) & 0xFFu);
+    return c;
+#endif
+}
+
 // x mod m for x < 2^24, 2 <= m <= 255, mu = floor(2^31 / m) (< 2^24 since m >= 129).
 // Barrett with q = floor(x mu / 2^31): q is floor(x/m) or one less, so r = x - q m
 // lies in [0, 2m) and one conditional subtraction finishes.
 OZ2_HD uint32_t mod24(uint32_t x, uint32_t m, uint32_t mu) {
     const uint64_t prod = (static_cast<uint64_t>(mulhi24(x, mu)) << 32) | mul24(x, mu);  // 48-bit x*mu
-    const uint32_t q = static_cast<uint32_t>(prod >> 31);
-    const uint32_t r = x - mul24(q, m);
+    const uint32_t q = static_cast<uint32_t>(prod >> 31);  // < 2^17
+    const uint32_t r = mad_i24(static_cast<int32_t>(q), -static_cast<int32_t>(m), x);
     return umin(r, r - m);
 }
 
@@ -71,46 +104,55 @@ OZ2_HD int32_t exponent_from_maxbits(uint64_t b, int tk) {
     return tk - frexp_exp_bits(b);
 }
 
-// Exact RNE(|v| * 2^e) for |v * 2^e| < 2^101 as a 128-bit magnitude (hpp:251-289).
+// Exact RNE(|v| * 2^e) for |v * 2^e| < 2^101 as a 128-bit magnitude (hpp:251-289),
+// in straight-line code: with sh < 0 the 53-bit mantissa moves right by
+// min(-sh, 63) and rounds to nearest even (all in lo); with sh >= 0 it moves
+// left by sh <= 100 across the two words.  Shift counts stay below 64.
 OZ2_HD void scaled_mag(uint64_t bits, int e, uint64_t& hi, uint64_t& lo) {
     const int bexp = static_cast<int>((bits >> 52) & 0x7FF);
-    uint64_t mant = bits & 0xFFFFFFFFFFFFFull;
-    if (bexp) mant |= 0x10000000000000ull;
+    const uint64_t mant = (bits & 0xFFFFFFFFFFFFFull) | (bexp ? 0x10000000000000ull : 0);
     const int sh = (bexp ? bexp : 1) - 1075 + e;
-    if (sh >= 0) {
-        if (sh >= 64) {
-            hi = mant << (sh - 64);
-            lo = 0;
-        } else {
-            lo = mant << sh;
-            hi = sh == 0 ? 0 : (mant >> (64 - sh));
-        }
-    } else {
-        const int rs = -sh < 63 ? -sh : 63;
-        const uint64_t q = mant >> rs;
-        const uint64_t half = 1ull << (rs - 1);
-        const uint64_t rem = mant & ((half << 1) - 1);
-        lo = q + ((rem > half || (rem == half && (q & 1))) ? 1 : 0);
-        hi = 0;
-    }
+    const int ls = sh > 0 ? sh : 0;
+    const int rs = sh < 0 ? (-sh < 63 ? -sh : 63) : 0;
+    const uint64_t q = mant >> rs;
+    const uint64_t rem = mant & ((1ull << rs) - 1);
+    const uint64_t half = rs ? (1ull << (rs - 1)) : ~0ull;
+    const uint64_t down = q + (rem + (q & 1) > half ? 1 : 0);
+    const uint64_t l0 = mant << (ls & 63);                       // ls < 64: low word
+    const uint64_t h0 = (mant >> 1) >> ((63 - ls) & 63);         // ls < 64: bits shifted out of it
+    lo = sh < 0 ? down : (ls < 64 ? l0 : 0);
+    hi = sh < 0 ? 0 : (ls < 64 ? h0 : l0);                       // ls >= 64: mant << (ls - 64)
 }
 
-// Limb forms of the offset-binary value u = 2^OFF + x (x = +-RNE(|v| 2^e)):
-//   Wide21   OFF = 104, five 21-bit limbs (tk > 62);
-//   Narrow21 OFF = 62, three 21-bit limbs (the top one <= 2^21);
-//   Narrow13 OFF = 62, five 13-bit limbs (the top one <= 2^11): more limbs, cheaper
-//            reduction (the limb sum stays below 2^23, see limb_residue).
-// Both narrow forms give the same residues; the split picks per memory order
-// (strided rows keep fewer registers live with Narrow21).
-enum class Limbs { Wide21, Narrow21, Narrow13 };
+// RNE(|v| * 2^e) for |v * 2^e| < 2^62 (narrow values) in straight-line code: the
+// mantissa moves left by max(sh, 0) and right by min(max(-sh, 0), 63) with the
+// round-to-nearest-even increment rem + lsb > half (half = 2^(rs-1); none if rs = 0).
+OZ2_HD uint64_t scaled_mag62(uint64_t bits, int e) {
+    const int bexp = static_cast<int>((bits >> 52) & 0x7FF);
+    const uint64_t mant = (bits & 0xFFFFFFFFFFFFFull) | (bexp ? 0x10000000000000ull : 0);
+    const int sh = (bexp ? bexp : 1) - 1075 + e;
+    const int ls = sh > 0 ? sh : 0;
+    const int rs = sh < 0 ? (-sh < 63 ? -sh : 63) : 0;
+    const uint64_t up = mant << ls;
+    const uint64_t q = up >> rs;
+    const uint64_t rem = up & ((1ull << rs) - 1);
+    const uint64_t half = rs ? (1ull << (rs - 1)) : ~0ull;
+    return q + (rem + (q & 1) > half ? 1 : 0);
+}
+
+// Octet forms of the offset-binary value u = 2^OFF + x (x = +-RNE(|v| 2^e)), kept
+// as little-endian 32-bit words:
+//   Octets8  OFF = 62 (tk <= 62): the 8 bytes of u <= 2^63 in a[0..1];
+//   Octets14 OFF = 104 (tk > 62): the 14 bytes of u <= 2^105 in a[0..3].
+enum class Limbs { Octets8, Octets14 };
 
 // zero: the element is outside the matrix or its row is non-finite, giving x = 0.
 template <Limbs F>
-OZ2_HD void to_limbs(uint64_t bits, int e, bool zero, uint32_t (&a)[5]) {
-    uint64_t hi = 0, lo = 0;
-    if (!zero) scaled_mag(bits, e, hi, lo);
+OZ2_HD void to_limbs(uint64_t bits, int e, bool zero, uint32_t (&a)[4]) {
     const bool neg = (bits >> 63) != 0 && !zero;
-    if (F == Limbs::Wide21) {  // mag <= 2^100
+    if (F == Limbs::Octets14) {  // mag <= 2^100
+        uint64_t hi = 0, lo = 0;
+        if (!zero) scaled_mag(bits, e, hi, lo);
         uint64_t uh, ul;
         if (neg) {
             ul = 0 - lo;
@@ -119,69 +161,47 @@ OZ2_HD void to_limbs(uint64_t bits, int e, bool zero, uint32_t (&a)[5]) {
             ul = lo;
             uh = (1ull << 40) + hi;
         }
-        constexpr uint64_t M21 = 0x1FFFFF;
-        a[0] = static_cast<uint32_t>(ul & M21);
-        a[1] = static_cast<uint32_t>((ul >> 21) & M21);
-        a[2] = static_cast<uint32_t>((ul >> 42) & M21);
-        a[3] = static_cast<uint32_t>(((ul >> 63) | (uh << 1)) & M21);
-        a[4] = static_cast<uint32_t>((uh >> 20) & M21);
-    } else {  // mag <= 2^62, u <= 2^63
+        a[0] = static_cast<uint32_t>(ul);
+        a[1] = static_cast<uint32_t>(ul >> 32);
+        a[2] = static_cast<uint32_t>(uh);
+        a[3] = static_cast<uint32_t>(uh >> 32);
+    } else {  // mag <= 2^62
+        const uint64_t lo = zero ? 0 : scaled_mag62(bits, e);
         const uint64_t ul = neg ? (1ull << 62) - lo : (1ull << 62) + lo;
-        if (F == Limbs::Narrow21) {
-            a[0] = static_cast<uint32_t>(ul & 0x1FFFFF);
-            a[1] = static_cast<uint32_t>((ul >> 21) & 0x1FFFFF);
-            a[2] = static_cast<uint32_t>(ul >> 42);
-            a[3] = a[4] = 0;
-        } else {
-            constexpr uint64_t M13 = 0x1FFF;
-            a[0] = static_cast<uint32_t>(ul & M13);
-            a[1] = static_cast<uint32_t>((ul >> 13) & M13);
-            a[2] = static_cast<uint32_t>((ul >> 26) & M13);
-            a[3] = static_cast<uint32_t>((ul >> 39) & M13);
-            a[4] = static_cast<uint32_t>(ul >> 52);
-        }
+        a[0] = static_cast<uint32_t>(ul);
+        a[1] = static_cast<uint32_t>(ul >> 32);
+        a[2] = a[3] = 0;
     }
 }
 
-// (x + h) mod m in [0, m) for modulus index t (m <= 255) from the limbs of u = 2^OFF + x.
-// 21-bit forms: v = a_0 + sum a_j (2^(21 j) mod m) + (h - 2^OFF mod m) < 2^32, then mod32.
-// Narrow13: v = a_0 + sum a_j (2^(13 j) mod m) + (h - 2^62 mod m) < 2^23, accumulated
-// doubled (2v < 2^24, constants pre-doubled) so that q = mulhi24(2v, mu) =
-// floor(v mu / 2^31) is the Barrett quotient (floor(v/m) or one less) and
-// r = (2v - q 2m) / 2 lies in [0, 2m).
-template <Limbs F>
-OZ2_HD uint32_t limb_residue(const uint32_t (&a)[5], const TabSet* __restrict__ tb, int t) {
-    if (F == Limbs::Narrow13) {
-        uint32_t v2 = 2 * a[0] + tb->off2[t];
-        v2 += mul24(a[1], tb->c13x2[t][0]);
-        v2 += mul24(a[2], tb->c13x2[t][1]);
-        v2 += mul24(a[3], tb->c13x2[t][2]);
-        v2 += mul24(a[4], tb->c13x2[t][3]);
-        const uint32_t q = mulhi24(v2, tb->mu[t]);
-        const uint32_t r = (v2 - mul24(q, tb->m2[t])) >> 1;
-        return umin(r, r - tb->m[t]);
-    }
-    constexpr bool W = F == Limbs::Wide21;
-    uint32_t v = a[0] + tb->off[W ? 1 : 0][t];
-    v += mul24(a[1], tb->c21[t][0]);
-    v += mul24(a[2], tb->c21[t][1]);
+// (x + h) mod m in [0, m) for modulus index t (m <= 255) from the octets of u = 2^OFF + x:
+// v = (h - 2^OFF mod m) + sum_j byte_j(u) (2^(8 j) mod m), one v_dot4 per word;
+// v <= 14 * 255 * 254 + 254 < 2^20, so q = mulhi24(2v, mu) = floor(v mu / 2^31) is
+// floor(v/m) or one less and r = v - q m lies in [0, 2m).
+template <Limbs F, typename Tab>
+OZ2_HD uint32_t limb_residue(const uint32_t (&a)[4], Tab tb, int t) {
+    constexpr bool W = F == Limbs::Octets14;
+    uint32_t v = udot4(a[0], tb->c8[t][0], tb->off[W ? 1 : 0][t]);
+    v = udot4(a[1], tb->c8[t][1], v);
     if (W) {
-        v += mul24(a[3], tb->c21[t][2]);
-        v += mul24(a[4], tb->c21[t][3]);
+        v = udot4(a[2], tb->c8[t][2], v);
+        v = udot4(a[3], tb->c8[t][3], v);
     }
-    return mod32(v, tb->m[t], tb->mu[t], tb->c16[t]);
+    const uint32_t q = mulhi24(v << 1, tb->mu[t]);  // < 2^13
+    const uint32_t r = mad_i24(static_cast<int32_t>(q), -static_cast<int32_t>(tb->m[t]), v);
+    return umin(r, r - tb->m[t]);
 }
 
 // int8 byte of the symmetric residue of x for modulus index t (m = 256 only at t = 0)
-template <Limbs F>
-OZ2_HD uint32_t sym_byte(const uint32_t (&a)[5], const TabSet* __restrict__ tb, int t) {
+template <Limbs F, typename Tab>
+OZ2_HD uint32_t sym_byte(const uint32_t (&a)[4], Tab tb, int t) {
     if (t == 0) return a[0] & 0xFFu;  // m = 256: x mod 256 = u mod 256
     return (limb_residue<F>(a, tb, t) - tb->h[t]) & 0xFFu;
 }
 
 // symmetric residue value in [-h, m-1-h] (complex split needs the value, not the byte)
-template <Limbs F>
-OZ2_HD int32_t sym_value(const uint32_t (&a)[5], const TabSet* __restrict__ tb, int t) {
+template <Limbs F, typename Tab>
+OZ2_HD int32_t sym_value(const uint32_t (&a)[4], Tab tb, int t) {
     if (t == 0) return static_cast<int32_t>((a[0] + 127u) & 0xFFu) - 127;
     return static_cast<int32_t>(limb_residue<F>(a, tb, t)) - static_cast<int32_t>(tb->h[t]);
 }
@@ -193,7 +213,8 @@ OZ2_HD int32_t sym_value(const uint32_t (&a)[5], const TabSet* __restrict__ tb, 
 OZ2_HD uint32_t residue_m(uint32_t x, uint32_t m, uint32_t mu, uint32_t c16, uint32_t c31) {
     return mod32(x + c31, m, mu, c16);
 }
-OZ2_HD uint32_t acc_residue(uint32_t x, const TabSet* __restrict__ tb, int t) {
+template <typename Tab>
+OZ2_HD uint32_t acc_residue(uint32_t x, Tab tb, int t) {
     const uint32_t m = tb->m[t];
     if (m == 256) return x & 0xFFu;
     return residue_m(x, m, tb->mu[t], tb->c16[t], tb->c31[t]);
@@ -218,19 +239,22 @@ struct CrtAcc {
         for (int l = 0; l < L; ++l) acc[l] = 0;
     }
     // u += r y_t/m_t (fp32); acc_l += r W16[t][l]  (r < 256, W16 < 2^16: products < 2^24, sums < 2^29)
-    OZ2_HD void add(const TabSet* __restrict__ tb, int t, uint32_t r) {
+    template <typename Tab>
+    OZ2_HD void add(Tab tb, int t, uint32_t r) {
         u = __builtin_fmaf(static_cast<float>(r), tb->frac[t], u);
         for (int l = 0; l < L; ++l) acc[l] += mul24(r, tb->W16[t][l]);
     }
     // moduli 2i and 2i+1 at once (r1 = 0 past s): one dot2 per limb with the paired table
-    OZ2_HD void add2(const TabSet* __restrict__ tb, int i, uint32_t r0, uint32_t r1) {
+    template <typename Tab>
+    OZ2_HD void add2(Tab tb, int i, uint32_t r0, uint32_t r1) {
         u = __builtin_fmaf(static_cast<float>(r0), tb->frac[2 * i], u);
         u = __builtin_fmaf(static_cast<float>(r1), tb->frac[2 * i + 1], u);
         const uint32_t rr = r0 | (r1 << 16);
         for (int l = 0; l < L; ++l) acc[l] = udot2(rr, tb->W16p[i][l], acc[l]);
     }
     // RNE(C') * 2^-shift with C' = sum r_t W_t - q M, q = rint(u)
-    OZ2_HD double finish(const TabSet* __restrict__ tb, int shift) const {
+    template <typename Tab>
+    OZ2_HD double finish(Tab tb, int shift) const {
         const int32_t q = static_cast<int32_t>(__builtin_rintf(u));
         uint32_t w[NL];
         int32_t carry = 0;

@@ -75,11 +75,8 @@ struct TabSet {
     uint32_t mu[kMaxModuli];                // floor(2^31 / m): Barrett constant for 24-bit operands
     uint32_t h[kMaxModuli];                 // (m - 1) / 2
     uint32_t c16[kMaxModuli];               // 2^16 mod m
-    uint32_t c21[kMaxModuli][4];            // 2^(21 j) mod m, j = 1..4
     uint32_t off[2][kMaxModuli];            // (m - 2^OFF mod m + h) mod m, OFF = 62 (narrow), 104 (wide)
-    uint32_t off2[kMaxModuli];              // 2 off[0] (narrow split, doubled accumulation)
-    uint32_t c13x2[kMaxModuli][4];          // 2 (2^(13 j) mod m), j = 1..4
-    uint32_t m2[kMaxModuli];                // 2 m
+    uint32_t c8[kMaxModuli][4];             // bytes 2^(8 j) mod m, j = 0..15, four to a word (low first)
     uint32_t c31[kMaxModuli];               // m * ceil(2^31 / m) (GEMM residue epilogue offset; 0 for m = 256)
     // CRT (hpp:222-236)
     float frac[kMaxModuli];                 // fp32(y_t / m_t)

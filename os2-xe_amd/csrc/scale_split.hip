@@ -8,8 +8,8 @@
 //
 // Both operands are written in one blocked plane layout (plane_offset: 1 KB
 // blocks of 16 rows x 64 k), so the GEMM copies A and B tiles the same way.
-// A thread's 16 (real) or 4 (complex) consecutive k stay inside one 16-byte
-// chunk of a block, so each plane store is one aligned vector store.  Rows of op(X)
+// A thread's 16 (real) or 4 (complex) k lie in one 16-byte chunk of a block
+// (block_k order), so each plane store is one aligned vector store.  Rows of op(X)
 // that are strided in memory (op(A) = N, op(B) = T) are read column-by-column
 // with lanes along rows, so no separate transpose pass is needed.
 #include <algorithm>
@@ -137,12 +137,22 @@ int64_t scale_kblocks(int64_t len, int64_t k) {
 // split
 // ---------------------------------------------------------------------------
 
-// Real split: each thread owns 16 consecutive k of one row -> one 16-byte store per plane.
+// Bit pattern of op(X)[r][p] for a row that exists (r < rows) and p < k.
+template <bool KCONTIG, bool CPLX>
+__device__ __forceinline__ uint64_t load_bits(const double* __restrict__ X, int64_t ldx, int64_t r, int64_t p, int part) {
+    const int64_t i = KCONTIG ? r * ldx + p : r + p * ldx;
+    return static_cast<uint64_t>(__double_as_longlong(CPLX ? X[2 * i + part] : X[i]));
+}
+
+// Real split: each thread owns one 16-byte chunk of one row (16 k, block_k order) ->
+// one 16-byte store per plane.
 // KCONTIG: row r is contiguous (X[p + r*ldx]); block = 256 chunks along the row(s).
 // !KCONTIG: rows strided (X[r + p*ldx]); block = 32 rows x 8 chunks (128 k).
+// All loads are issued before any arithmetic and stay inside the matrix: padding
+// rows read row 0 and k past the end reads k - 1, and those values are then zeroed.
 template <bool WIDE, bool KCONTIG>
 __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, uint32_t bx, uint32_t by) {
-    const TabSet* __restrict__ tb = a.tab;
+    const auto tb = const_tab(a.tab);
     int64_t r, ck;
     if (KCONTIG) {
         const int tpr = 1 << lg_tpr;  // threads per row in a block
@@ -154,27 +164,43 @@ __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, 
     }
     const int64_t nck = a.kp >> 4;
     if (r >= a.rows_pad || ck >= nck) return;
-    const int64_t p0 = ck * 16;
+    const int64_t blk = ck >> 2;
+    const int c = static_cast<int>(ck & 3);
+    const int64_t kb = blk * 64 + 2 * c;  // byte q of the chunk holds k = kb + 8 (q >> 1) + (q & 1)
     const bool rowok = r < a.rows;
-    const int32_t e = rowok ? a.ex[r] : 0;
-    const bool rowzero = !rowok || e == INT32_MIN;
+    const int32_t ex = rowok ? a.ex[r] : 0;
+    const bool rowzero = !rowok || ex == INT32_MIN;
+    const int e = rowzero ? 0 : ex;
+    const int64_t rl = rowok ? r : 0;
 
-    // narrow values: 13-bit limbs when rows are contiguous, 21-bit limbs (fewer live
-    // registers for the strided loads) when they are not
-    constexpr Limbs F = WIDE ? Limbs::Wide21 : KCONTIG ? Limbs::Narrow13 : Limbs::Narrow21;
-    uint32_t limbs[16][5];
+    uint64_t bits[16];
+    const bool full = blk * 64 + 64 <= a.k;
+    const double* rowp = a.X + rl * a.ldx;
+    if (KCONTIG && full && (reinterpret_cast<uintptr_t>(rowp + kb) & 15) == 0) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int64_t p = p0 + q;
-        uint64_t bits = 0;
-        const bool z = rowzero || p >= a.k;
-        if (!z) {
-            const double* src = KCONTIG ? a.X + r * a.ldx + p : a.X + r + p * a.ldx;
-            bits = static_cast<uint64_t>(__double_as_longlong(*src));
+        for (int i = 0; i < 8; ++i) {
+            const uint4 v = *reinterpret_cast<const uint4*>(rowp + kb + 8 * i);
+            bits[2 * i] = static_cast<uint64_t>(v.x) | (static_cast<uint64_t>(v.y) << 32);
+            bits[2 * i + 1] = static_cast<uint64_t>(v.z) | (static_cast<uint64_t>(v.w) << 32);
         }
-        to_limbs<F>(bits, e, z, limbs[q]);
+    } else if (full) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) bits[q] = load_bits<KCONTIG, false>(a.X, a.ldx, rl, kb + 8 * (q >> 1) + (q & 1), 0);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int64_t p = kb + 8 * (q >> 1) + (q & 1);
+            bits[q] = load_bits<KCONTIG, false>(a.X, a.ldx, rl, p < a.k ? p : a.k - 1, 0);
+            bits[q] = p < a.k ? bits[q] : 0;
+        }
     }
-    int8_t* out = a.P + plane_offset(r, p0, a.kp);
+
+    // narrow values: the 8 bytes of the offset value, wide ones: its 14 bytes
+    constexpr Limbs F = WIDE ? Limbs::Octets14 : Limbs::Octets8;
+    uint32_t limbs[16][4];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) to_limbs<F>(rowzero ? 0 : bits[q], e, false, limbs[q]);
+    int8_t* out = a.P + chunk_offset(r, blk, c, a.kp);
     for (int t = 0; t < a.s; ++t) {
         uint32_t w[4];
 #pragma unroll
@@ -189,10 +215,11 @@ __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, 
 }
 
 // Complex split: three plane groups Re, Im (negated for conj), Re+Im, each
-// reduced to the symmetric range [-h, m-1-h].  4 consecutive k per thread.
+// reduced to the symmetric range [-h, m-1-h].  4 bytes of one chunk per thread
+// (k = kb, kb + 1, kb + 8, kb + 9 in block_k order).
 template <bool WIDE, bool KCONTIG, bool CONJ>
 __device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, uint32_t bx, uint32_t by) {
-    const TabSet* __restrict__ tb = a.tab;
+    const auto tb = const_tab(a.tab);
     int64_t r, ck;
     if (KCONTIG) {
         const int tpr = 1 << lg_tpr;
@@ -204,27 +231,34 @@ __device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, 
     }
     const int64_t nck = a.kp >> 2;
     if (r >= a.rows_pad || ck >= nck) return;
-    const int64_t p0 = ck * 4;
+    const int64_t blk = ck >> 4;
+    const int c = static_cast<int>((ck >> 2) & 3), w = static_cast<int>(ck & 3);
+    const int64_t kb = blk * 64 + 16 * w + 2 * c;  // bytes 4w + q: k = kb + 8 (q >> 1) + (q & 1)
     const bool rowok = r < a.rows;
-    const int32_t e = rowok ? a.ex[r] : 0;
-    const bool rowzero = !rowok || e == INT32_MIN;
+    const int32_t ex = rowok ? a.ex[r] : 0;
+    const bool rowzero = !rowok || ex == INT32_MIN;
+    const int e = rowzero ? 0 : ex;
+    const int64_t rl = rowok ? r : 0;
 
-    constexpr Limbs F = WIDE ? Limbs::Wide21 : KCONTIG ? Limbs::Narrow13 : Limbs::Narrow21;
-    uint32_t lr[4][5], li[4][5];
+    uint64_t br[4], bi[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int64_t p = p0 + q;
-        uint64_t br = 0, bi = 0;
+        const int64_t p = kb + 8 * (q >> 1) + (q & 1);
+        const int64_t pc = p < a.k ? p : a.k - 1;
+        br[q] = load_bits<KCONTIG, true>(a.X, a.ldx, rl, pc, 0);
+        bi[q] = load_bits<KCONTIG, true>(a.X, a.ldx, rl, pc, 1);
         const bool z = rowzero || p >= a.k;
-        if (!z) {
-            const double* src = KCONTIG ? a.X + 2 * (r * a.ldx + p) : a.X + 2 * (r + p * a.ldx);
-            br = static_cast<uint64_t>(__double_as_longlong(src[0]));
-            bi = static_cast<uint64_t>(__double_as_longlong(src[1]));
-        }
-        to_limbs<F>(br, e, z, lr[q]);
-        to_limbs<F>(bi, e, z, li[q]);
+        br[q] = z ? 0 : br[q];
+        bi[q] = z ? 0 : bi[q];
     }
-    int8_t* out = a.P + plane_offset(r, p0, a.kp);
+    constexpr Limbs F = WIDE ? Limbs::Octets14 : Limbs::Octets8;
+    uint32_t lr[4][4], li[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        to_limbs<F>(br[q], e, false, lr[q]);
+        to_limbs<F>(bi[q], e, false, li[q]);
+    }
+    int8_t* out = a.P + chunk_offset(r, blk, c, a.kp) + 4 * w;
     for (int t = 0; t < a.s; ++t) {
         const int32_t m = static_cast<int32_t>(tb->m[t]);
         const int32_t h = static_cast<int32_t>(tb->h[t]);

@@ -75,12 +75,11 @@ __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __rest
         const double v = contig ? X[pp + i * ld] : X[i + pp * ld];
         return static_cast<uint64_t>(__double_as_longlong(v)) & 0x7FFFFFFFFFFFFFFFull;
     };
-    __shared__ unsigned int smax, sgmax, swide;
+    __shared__ unsigned int smax, sgmax;
     __shared__ unsigned long long scnt[kBins + 2];
     if (threadIdx.x == 0) {
         smax = 0;
         sgmax = 0;
-        swide = 0;
     }
     if (threadIdx.x < kBins + 2) scnt[threadIdx.x] = 0;
     __syncthreads();

@@ -21,7 +21,7 @@ template <math::Limbs F>
 void split_form(const double* v, const int32_t* e, int64_t n, int s, int8_t* planes) {
     const TabSet* tb = tables() + s;
     for (int64_t i = 0; i < n; ++i) {
-        uint32_t a[5];
+        uint32_t a[4];
         uint64_t bits;
         __builtin_memcpy(&bits, v + i, 8);
         const bool zero = e[i] == INT32_MIN;
@@ -35,7 +35,7 @@ void split_cplx_form(const double* re, const double* im, const int32_t* e, int64
                      int8_t* planes) {
     const TabSet* tb = tables() + s;
     for (int64_t i = 0; i < n; ++i) {
-        uint32_t ar[5], ai[5];
+        uint32_t ar[4], ai[4];
         uint64_t br, bi;
         __builtin_memcpy(&br, re + i, 8);
         __builtin_memcpy(&bi, im + i, 8);
@@ -63,19 +63,17 @@ void split_cplx_form(const double* re, const double* im, const int32_t* e, int64
 extern "C" {
 
 // symmetric int8 residues of RNE(v_i * 2^e_i): planes[t * n + i]; e = INT32_MIN gives 0.
-// narrow13: for tk <= 62, the 13-bit limb form (else the 21-bit one); ignored when wide
-void emu_split(const double* v, const int32_t* e, int64_t n, int s, int tk, int narrow13, int8_t* planes) {
-    if (tk > kNarrowScaleBits) split_form<math::Limbs::Wide21>(v, e, n, s, planes);
-    else if (narrow13) split_form<math::Limbs::Narrow13>(v, e, n, s, planes);
-    else split_form<math::Limbs::Narrow21>(v, e, n, s, planes);
+// the octets of u = 2^OFF + x: 8 bytes for tk <= 62, else 14
+void emu_split(const double* v, const int32_t* e, int64_t n, int s, int tk, int8_t* planes) {
+    if (tk > kNarrowScaleBits) split_form<math::Limbs::Octets14>(v, e, n, s, planes);
+    else split_form<math::Limbs::Octets8>(v, e, n, s, planes);
 }
 
 // complex: groups Re, Im (negated if conj), Re+Im reduced to [-h, m-1-h]: planes[(g*s + t)*n + i]
 void emu_split_cplx(const double* re, const double* im, const int32_t* e, int64_t n, int s, int tk, int conj,
-                    int narrow13, int8_t* planes) {
-    if (tk > kNarrowScaleBits) split_cplx_form<math::Limbs::Wide21>(re, im, e, n, s, conj, planes);
-    else if (narrow13) split_cplx_form<math::Limbs::Narrow13>(re, im, e, n, s, conj, planes);
-    else split_cplx_form<math::Limbs::Narrow21>(re, im, e, n, s, conj, planes);
+                    int8_t* planes) {
+    if (tk > kNarrowScaleBits) split_cplx_form<math::Limbs::Octets14>(re, im, e, n, s, conj, planes);
+    else split_cplx_form<math::Limbs::Octets8>(re, im, e, n, s, conj, planes);
 }
 
 // GEMM epilogue: residues of int32 sums, out[t * n + i] in [0, m_t)

@@ -19,8 +19,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLD = os.path.join(ROOT, "tests", "golden")
 _emu = ctypes.CDLL(os.path.join(ROOT, "os2-xe_amd", "lib", "liboz2_emu.so"))
 vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
-_emu.emu_split.argtypes = [vp, vp, i64, i32, i32, i32, vp]
-_emu.emu_split_cplx.argtypes = [vp, vp, vp, i64, i32, i32, i32, i32, vp]
+_emu.emu_split.argtypes = [vp, vp, i64, i32, i32, vp]
+_emu.emu_split_cplx.argtypes = [vp, vp, vp, i64, i32, i32, i32, vp]
 _emu.emu_acc_residue.argtypes = [vp, i64, i32, vp]
 _emu.emu_crt.argtypes = [vp, vp, i64, i32, vp]
 _emu.emu_exponent_from_maxbits.argtypes = [ctypes.c_uint64, i32]
@@ -39,28 +39,25 @@ def h2f(h):
     return struct.unpack("<d", struct.pack("<Q", h))[0]
 
 
-def emu_split(v, e, s, tk, narrow13=0):
-    """narrow13: the 13-bit limb form of narrow values (tk <= 62), else the 21-bit one"""
+def emu_split(v, e, s, tk):
     v = np.ascontiguousarray(v, dtype=np.float64)
     e = np.ascontiguousarray(e, dtype=np.int32)
     out = np.zeros((s, v.size), dtype=np.int8)
-    _emu.emu_split(P(v), P(e), v.size, s, tk, narrow13, P(out))
+    _emu.emu_split(P(v), P(e), v.size, s, tk, P(out))
     return out
 
 
-@pytest.mark.parametrize("narrow13", [0, 1])
-def test_split_golden(narrow13):
+def test_split_golden():
     for case in load("split.json"):
         for col in case["cols"]:
             v = np.array([h2f(b) for b in col["bits"]])
             e = np.full(v.size, col["e"], dtype=np.int32)
-            got = emu_split(v, e, case["s"], case["tk"], narrow13)
+            got = emu_split(v, e, case["s"], case["tk"])
             assert got.tolist() == col["res"], (case["s"], case["tk"])
 
 
-@pytest.mark.parametrize("narrow13", [0, 1])
 @pytest.mark.parametrize("s,tk", [(2, 6), (8, 24), (13, 44), (16, 55), (18, 62), (19, 63), (24, 86), (24, 100)])
-def test_split_random_vs_oracle(s, tk, narrow13, rng):
+def test_split_random_vs_oracle(s, tk, rng):
     n = 20000
     v = rng.uniform(-1, 1, n) * np.exp2(rng.integers(-70, 1, n).astype(float))
     v[:8] = [0.0, -0.0, 5e-324, -5e-324, 1.0 - 2 ** -53, -(1.0 - 2 ** -53), 0.5, -0.75]
@@ -68,22 +65,21 @@ def test_split_random_vs_oracle(s, tk, narrow13, rng):
     e = np.full(n, tk - math.frexp(mx)[1], dtype=np.int32)
     e[:50] = rng.integers(tk - 60, tk + 1, 50)  # other exponents, |v 2^e| < 2^tk still
     e[50] = -2 ** 31  # non-finite row: all residues zero
-    got = emu_split(v, e, s, tk, narrow13)
+    got = emu_split(v, e, s, tk)
     want = O.split_values(v, np.where(e == -2 ** 31, 0, e), s)
     want[:, 50] = 0
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("narrow13", [0, 1])
 @pytest.mark.parametrize("conj", [0, 1])
 @pytest.mark.parametrize("s,tk", [(14, 49), (16, 55), (24, 86)])
-def test_split_complex_vs_oracle(s, tk, conj, narrow13, rng):
+def test_split_complex_vs_oracle(s, tk, conj, rng):
     n = 5000
     re = rng.uniform(-1, 1, n)
     im = rng.uniform(-1, 1, n)
     e = np.full(n, tk, dtype=np.int32)
     out = np.zeros((3, s, n), dtype=np.int8)
-    _emu.emu_split_cplx(P(re), P(im), P(e), n, s, tk, conj, narrow13, P(out))
+    _emu.emu_split_cplx(P(re), P(im), P(e), n, s, tk, conj, P(out))
     r0 = O.split_values(re, e, s).astype(np.int64)
     r1 = O.split_values(im, e, s).astype(np.int64)
     for t, m in enumerate(O.moduli(s)):

@@ -32,11 +32,14 @@ def h2f(h):
 
 def plane_index(rows_pad, kp):
     """flat offset of (row, k) in one blocked plane: 16-row bands of 1 KB blocks (64 k each),
-    64 bytes per row, 16-byte k-chunks XOR-permuted by (row >> 2) & 3 (oz2_c.h, oz2_unit_split)"""
+    64 bytes per row as four 16-byte chunks; chunk c holds the k pairs (2c, 2c+1) of each
+    group of 8 k, and chunks are XOR-permuted by (row >> 2) & 3 (oz2_c.h, oz2_unit_split)"""
     r = np.arange(rows_pad)[:, None]
     p = np.arange(kp)[None, :]
-    return ((((r >> 4) * (kp >> 6) + (p >> 6)) << 10) + ((r & 15) << 6) + ((((p >> 4) & 3) ^ ((r >> 2) & 3)) << 4)
-            + (p & 15))
+    q = p & 63
+    c = (q >> 1) & 3
+    b = ((q >> 3) << 1) | (q & 1)
+    return (((r >> 4) * (kp >> 6) + (p >> 6)) << 10) + ((r & 15) << 6) + ((c ^ ((r >> 2) & 3)) << 4) + b
 
 
 def run_split(values, ex, s, tk, kcontig, cplx=False, conj=False):
