@@ -31,7 +31,7 @@ __device__ __forceinline__ uint32_t chunk_sum(uint32_t s, uint32_t m) {
 // 4 consecutive rows per thread; residues are bytes packed 4 per 32-bit load.
 template <int NL, bool CPLX>
 __device__ __forceinline__ void crt_body(const CrtArgs& a, uint32_t bx) {
-    const TabSet* __restrict__ tb = a.tab;
+    const auto tb = const_tab(a.tab);
     const int64_t mq = (a.mb + 3) >> 2;
     const int64_t idx = static_cast<int64_t>(bx) * 256 + threadIdx.x;
     const int64_t j = idx / mq;
@@ -48,8 +48,24 @@ __device__ __forceinline__ void crt_body(const CrtArgs& a, uint32_t bx) {
         re[e].init();
         if (CPLX) im[e].init();
     }
+    if (!CPLX && a.kch == 1) {
+        // one k-chunk: the s residue words (4 entries each) are all loaded before any
+        // arithmetic, then summed in modulus pairs (2 residues per dot2 step); r = 0 past s
+        uint32_t w[kMaxModuli];
+#pragma unroll
+        for (int t = 0; t < kMaxModuli; ++t)
+            w[t] = t < s ? *reinterpret_cast<const uint32_t*>(base + t * a.plane_r) : 0u;
+#pragma unroll
+        for (int t = 0; t < kMaxModuli; t += 2) {
+            if (t < s) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    re[e].add2(tb, t >> 1, (w[t] >> (8 * e)) & 0xFFu, (w[t + 1] >> (8 * e)) & 0xFFu);
+            }
+        }
+    }
     // moduli in pairs (2 residues per dot2 step of the limb sums); r = 0 past s
-    for (int t = 0; t < s; t += 2) {
+    for (int t = 0; t < s && (CPLX || a.kch > 1); t += 2) {
         const bool two = t + 1 < s;
         const uint32_t m0 = tb->m[t], m1 = two ? tb->m[t + 1] : 1u;
         if (!CPLX) {

@@ -259,7 +259,8 @@ struct CrtAcc {
         uint32_t w[NL];
         int32_t carry = 0;
         for (int l = 0; l < L; ++l) {
-            const int32_t v = static_cast<int32_t>(acc[l]) - q * static_cast<int32_t>(tb->M16[l]) + carry;
+            // 0 <= q <= s * 255 and M16 < 2^16: one v_mad_i32_i24
+            const int32_t v = static_cast<int32_t>(mad_i24(-q, static_cast<int32_t>(tb->M16[l]), acc[l] + static_cast<uint32_t>(carry)));
             const uint32_t lo = static_cast<uint32_t>(v) & 0xFFFFu;
             if (l & 1) w[l >> 1] |= lo << 16;
             else w[l >> 1] = lo;
