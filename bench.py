@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--vendor-int8", action="store_true",
+                    help="also time the vendor INT8 GEMM (hipBLASLt through torch._int_mm) on s GEMMs of the same "
+                         "shape, int32 output, as the rate this GPU reaches with a library kernel")
     ap.add_argument("--workload", default="dgemm8192",
                     choices=["dgemm8192", "zgemm4096", "wide4000", "ksplit", "batch"],
                     help="dgemm8192 (default line), or one of the other measured configurations")
@@ -307,6 +310,34 @@ def cpu_baseline(m, n, k, s):
                       f"oracle residue route (split, exact int8 GEMM, CRT) with OpenMP"}
 
 
+def vendor_int8(torch, m, n, k, s, reps=5):
+    """hipBLASLt int8 x int8 -> int32 (torch._int_mm) on s GEMMs of m x n x k, best of reps; op(B) = T and N."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    A = [torch.randint(-128, 128, (m, k), dtype=torch.int8, device="cuda", generator=g) for _ in range(2)]
+    out = {"library": "hipBLASLt (torch._int_mm), int32 output", "gemms": s}
+    for layout in ("NT", "NN"):
+        B = [torch.randint(-128, 128, (n, k) if layout == "NT" else (k, n), dtype=torch.int8, device="cuda",
+                           generator=g) for _ in range(2)]
+        B = [b.t() for b in B] if layout == "NT" else B
+        torch._int_mm(A[0], B[0])
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for i in range(s):
+                torch._int_mm(A[i & 1], B[i & 1])
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        out[f"{layout}_ms"] = round(best, 4)
+        out[f"{layout}_tops"] = round(2.0 * m * n * k * s / best / 1e9, 1)
+        del B
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -412,6 +443,8 @@ def main():
                     ts.append(time.perf_counter() - a)
                 sweep[f"s{sv}_{ta}{tb}"] = round(flops / min(ts) / 1e12, 2)
         out["sweep_tflops"] = sweep
+    if args.vendor_int8:
+        out["roofline"]["vendor_int8"] = vendor_int8(torch, m, n, k, s)
     if not args.no_batch:
         del dA, dB, dC, dC0
         torch.cuda.empty_cache()

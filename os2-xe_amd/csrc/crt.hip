@@ -123,6 +123,34 @@ __device__ __forceinline__ void crt_body(const CrtArgs& a, uint32_t bx) {
             }
         }
     }
+    if (!CPLX && i0 + 4 <= a.mb) {
+        // four entries of one column: 16-byte accesses to C when it is aligned for them
+        double v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int32_t Di = a.D[i0 + e];
+            const bool nonfinite = Di == INT32_MIN || Ej == INT32_MIN;
+            v[e] = re[e].finish(tb, nonfinite ? 0 : Di + Ej);
+            if (nonfinite) v[e] = __builtin_nan("");
+        }
+        double* cp = a.C + i0 + j * a.ldc;
+        if ((reinterpret_cast<uintptr_t>(cp) & 15) == 0) {
+            double2* c2 = reinterpret_cast<double2*>(cp);
+            if (a.br == 0.0) {
+                c2[0] = make_double2(a.ar * v[0], a.ar * v[1]);
+                c2[1] = make_double2(a.ar * v[2], a.ar * v[3]);
+            } else {
+                const double2 x0 = c2[0], x1 = c2[1];
+                c2[0] = make_double2(__builtin_fma(a.ar, v[0], a.br * x0.x), __builtin_fma(a.ar, v[1], a.br * x0.y));
+                c2[1] = make_double2(__builtin_fma(a.ar, v[2], a.br * x1.x), __builtin_fma(a.ar, v[3], a.br * x1.y));
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                cp[e] = a.br == 0.0 ? a.ar * v[e] : __builtin_fma(a.ar, v[e], a.br * cp[e]);
+        }
+        return;
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int64_t i = i0 + e;
