@@ -178,7 +178,8 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
                                   "s_hist": {str(v): s_sel.count(v) for v in sorted(set(s_sel))},
                                   "launches_per_run": info["launches"], "workspace_gib": round(info["workspace_bytes"] / 2 ** 30, 2),
                                   "autotuned": {"persistent": tune["persistent"], "largest_first": tune["largest_first"],
-                                                "tiles": {str(t): tune["tiles"].count(t) for t in (64, 128, 256)},
+                                                "tiles": {("256x128" if t == 256128 else str(t)): tune["tiles"].count(t)
+                                                          for t in (64, 128, 256, 256128)},
                                                 "tune_ms_no_graph": round(tune["ms"], 3)}},
         "speedup_vs_plain": round(t_plain / t_batch, 3),
     }

@@ -91,8 +91,9 @@ int oz2_moduli(int num_moduli, uint16_t* out);
  * ten launches for any count); with OZ2_BATCH_GRAPH that sequence is captured in a hipGraph when the
  * plan is created and oz2_batch_run replays it.  Buffers are bound at creation: running the plan
  * again recomputes from their current contents.  Each problem's C is bit-identical to oz2_dgemm with
- * the same num_moduli.  tiles: NULL (per-problem heuristic) or count entries of 256, 128, 64 or 0
- * (heuristic).  oz2_batch_run enqueues on the handle's current stream; oz2_batch_destroy waits for
+ * the same num_moduli.  tiles: NULL (per-problem heuristic) or count macro-tile codes: 256, 128, 64
+ * (square tiles of that many rows and columns), 256128 (256 rows of op(A) x 128 columns of op(B),
+ * two workgroups per CU: faster for k up to about 1024) or 0 (heuristic).  oz2_batch_run enqueues on the handle's current stream; oz2_batch_destroy waits for
  * the device first.  Real DGEMMs only. */
 typedef struct {
     int ta, tb;

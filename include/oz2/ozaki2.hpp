@@ -37,7 +37,7 @@ struct Tuning {
     bool batched_gemm = true;
     bool pad_lda = true;
     bool pow2_tiles = true;
-    int gemm_tile = -1;  // MI355X extension: -1 auto, 256 / 128 / 64 forces the INT8 GEMM macro tile
+    int gemm_tile = -1;  // MI355X extension: -1 auto, 256 / 128 / 64 / 256128 (256 x 128) forces the INT8 GEMM macro tile
 };
 
 struct TileInfo {

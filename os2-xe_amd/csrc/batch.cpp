@@ -15,7 +15,6 @@ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string("oz2: ") + what + ": " + hipGetErrorString(e));
 }
-k::Tile tile_of_rows(int t) { return t == 256 ? k::Tile::T256 : t == 128 ? k::Tile::T128 : k::Tile::T64; }
 }  // namespace
 
 BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector<int>& s,
@@ -24,8 +23,12 @@ BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector
     const size_t np = d_.size();
     if (s.size() != np) throw std::invalid_argument("oz2: batch needs one number of moduli per problem");
     if (!tiles.empty() && tiles.size() != np) throw std::invalid_argument("oz2: batch needs one tile per problem");
+    k::Tile unused;
+    for (int t : tiles)
+        if (t > 0 && !k::tile_from_code(t, unused))
+            throw std::invalid_argument("oz2: batch tile must be 64, 128, 256 or 256128 (0: automatic)");
     s_ = s;
-    tile_rows_.assign(np, 0);
+    tile_codes_.assign(np, 0);
     // validation (hpp:745-748, per problem)
     for (size_t i = 0; i < np; ++i) {
         const GemmDesc& g = d_[i];
@@ -41,7 +44,8 @@ BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector
     // per-problem layout inside one workspace: D, E, partial maxima, planes, residues
     struct P {
         Plan plan;
-        int64_t T, kp, kch, mpad, npad, planeA, planeB, planeR;
+        k::Tile tile;
+        int64_t kp, kch, mpad, npad, planeA, planeB, planeR;
         size_t oD, oE, oPA, oPB, oA, oB, oR;
     };
     std::vector<P> pp(np);
@@ -62,13 +66,13 @@ BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector
         P& q = pp[i];
         q.plan = make_plan(s_[i], g.k);  // throws invalid_argument when s is too small for k (hpp:186)
         const int planes = s_[i];
-        const k::Tile t = tiles.empty() || tiles[i] <= 0 ? eng_.pick_tile(g.m, g.n, planes) : tile_of_rows(tiles[i]);
-        q.T = k::tile_rows(t);
-        tile_rows_[i] = static_cast<int>(q.T);
         q.kp = round_up(g.k, kKStep);
         q.kch = (q.kp + kMaxKChunk - 1) / kMaxKChunk;
-        q.mpad = round_up(g.m, q.T);
-        q.npad = round_up(g.n, q.T);
+        if (tiles.empty() || tiles[i] <= 0) q.tile = eng_.pick_tile(g.m, g.n, std::min(q.kp, kMaxKChunk), planes);
+        else k::tile_from_code(tiles[i], q.tile);
+        tile_codes_[i] = k::tile_code(q.tile);
+        q.mpad = round_up(g.m, k::tile_m(q.tile));
+        q.npad = round_up(g.n, k::tile_n(q.tile));
         q.planeA = q.mpad * q.kp;
         q.planeB = q.npad * q.kp;
         q.planeR = q.npad * q.mpad;
@@ -166,12 +170,12 @@ BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector
                 a.k0 = c * kMaxKChunk;
                 a.klen = std::min(kMaxKChunk, q.kp - a.k0);
                 a.R = reinterpret_cast<uint8_t*>(base + q.oR) + c * s_[i] * q.planeR;
-                a.tiles_m = static_cast<int>(q.mpad / q.T);
-                a.tiles_n = static_cast<int>(q.npad / q.T);
+                a.tiles_m = static_cast<int>(q.mpad / k::tile_m(q.tile));
+                a.tiles_n = static_cast<int>(q.npad / k::tile_n(q.tile));
                 a.nplanes = s_[i];
                 a.s = s_[i];
                 a.tab = tabs + s_[i];
-                gj.push_back({tile_of_rows(static_cast<int>(q.T)), a});
+                gj.push_back({q.tile, a});
             }
         }
         scale_.clear();
@@ -304,11 +308,14 @@ TuneResult autotune_batch(Engine& eng, const std::vector<GemmDesc>& d, const std
         }
         return static_cast<double>(bestms);
     };
-    auto policy = [&](int64_t a, int64_t b) {
+    // square tile by min(m, n); problems with k loops up to kr (and room for the rows)
+    // take 256 x 128 tiles instead of the square ones
+    auto policy = [&](int64_t a, int64_t b, int64_t kr) {
         std::vector<int> t(d.size());
         for (size_t i = 0; i < d.size(); ++i) {
             const int64_t x = std::min(d[i].m, d[i].n);
             t[i] = x >= a ? 256 : x >= b ? 128 : 64;
+            if (t[i] != 64 && round_up(d[i].k, kKStep) <= kr && d[i].m > 128 && d[i].n > 64) t[i] = 256128;
         }
         return t;
     };
@@ -318,11 +325,12 @@ TuneResult autotune_batch(Engine& eng, const std::vector<GemmDesc>& d, const std
         std::vector<int> tiles;
     };
     std::vector<Cand> cands{{"heuristic", {}}};
-    for (auto [a, b] : {std::pair<int64_t, int64_t>{192, 64}, {256, 96}, {384, 128}, {512, 160}, {768, 256},
-                        {inf, 128}, {inf, 64}})
-        cands.push_back({"tile256>=" + (a >= inf ? std::string("inf") : std::to_string(a)) + ",tile128>=" +
-                             std::to_string(b),
-                         policy(a, b)});
+    for (int64_t kr : {int64_t{0}, int64_t{1024}})
+        for (auto [a, b] : {std::pair<int64_t, int64_t>{192, 64}, {256, 96}, {384, 128}, {512, 160}, {768, 256},
+                            {inf, 128}, {inf, 64}})
+            cands.push_back({"tile256>=" + (a >= inf ? std::string("inf") : std::to_string(a)) + ",tile128>=" +
+                                 std::to_string(b) + (kr ? ",tile256x128 for k<=" + std::to_string(kr) : ""),
+                             policy(a, b, kr)});
     try {
         for (const Cand& c : cands)
             for (bool pers : {false, true}) {

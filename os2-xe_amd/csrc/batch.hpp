@@ -45,7 +45,8 @@ struct BatchOptions {
 class BatchPlan {
 public:
     // s[i] > 0 fixes problem i's moduli; the engine supplies tables, stream and device facts.
-    // tiles[i] in {256, 128, 64} forces a macro tile (empty: engine heuristic per problem).
+    // tiles[i] in {256, 128, 64, 256128} forces a macro tile (k::tile_code; 0 or empty:
+    // engine heuristic per problem).
     BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector<int>& s, const std::vector<int>& tiles,
               bool use_graph, BatchOptions opt = {});
     ~BatchPlan();
@@ -55,7 +56,7 @@ public:
     void run(hipStream_t st);           // enqueue the whole batch on st
     void run_gemm_only(hipStream_t st); // the grouped INT8 GEMM launches alone (autotuning)
     const std::vector<int>& moduli() const { return s_; }
-    const std::vector<int>& tiles() const { return tile_rows_; }
+    const std::vector<int>& tiles() const { return tile_codes_; }  // k::tile_code per problem (0: no GEMM)
     size_t workspace_bytes() const { return ws_size_; }
     int launches() const;               // kernel launches per run()
     bool uses_graph() const { return exec_ != nullptr; }
@@ -65,7 +66,7 @@ private:
     Engine& eng_;
     std::vector<GemmDesc> d_;
     BatchOptions opt_;
-    std::vector<int> s_, tile_rows_;
+    std::vector<int> s_, tile_codes_;
     std::vector<size_t> special_;  // k == 0 or alpha == 0: C = beta C
     std::vector<k::GroupedLaunch> scale_, split_, gemm_, crt_;
     std::byte* ws_ = nullptr;
@@ -78,8 +79,9 @@ private:
 // Tile shapes and schedule for one batch, chosen by measuring whole runs of candidate
 // plans (the union of all problems fills the GPU, so the per-problem heuristic is not
 // the best choice): tile policies tile(min(m, n)) = 256 if >= a, 128 if >= b, else 64,
-// plus the single-problem heuristic, each with and without the persistent schedule;
-// then problem order instead of largest first for the winner.
+// each also with 256 x 128 tiles for the problems whose k loop is at most 1024, plus
+// the single-problem heuristic, each with and without the persistent schedule; then
+// problem order instead of largest first for the winner.
 struct TuneResult {
     std::vector<int> tiles;
     BatchOptions opt;

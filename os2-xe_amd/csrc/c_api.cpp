@@ -266,8 +266,9 @@ int oz2_set_tuning(oz2_handle h, const oz2_tuning_t* t) {
     return guard([&] {
         need(h);
         if (!t) throw std::invalid_argument("oz2: null tuning");
-        if (t->gemm_tile != -1 && t->gemm_tile != 64 && t->gemm_tile != 128 && t->gemm_tile != 256)
-            throw std::invalid_argument("oz2: gemm_tile must be -1, 64, 128 or 256");
+        oz2::k::Tile tile;
+        if (t->gemm_tile != -1 && !oz2::k::tile_from_code(t->gemm_tile, tile))
+            throw std::invalid_argument("oz2: gemm_tile must be -1, 64, 128, 256 or 256128");
         oz2::Tuning u;
         u.batched_gemm = t->batched_gemm != 0;
         u.pad_lda = t->pad_lda != 0;

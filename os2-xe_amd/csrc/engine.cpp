@@ -26,7 +26,8 @@ constexpr double kLaunch = 6e-6;
 }  // namespace
 
 struct Engine::Layout {
-    int64_t T = 256, kp = 0, mpad = 0, npad = 0, kch = 1;
+    k::Tile tile = k::Tile::T256;
+    int64_t TM = 256, TN = 256, kp = 0, mpad = 0, npad = 0, kch = 1;
     int g = 1, planes = 1;
     int64_t planeA = 0, planeB = 0, planeR = 0;
     size_t oD = 0, oE = 0, oPA = 0, oPB = 0, oA = 0, oB = 0, oR = 0, total = 0;
@@ -149,21 +150,20 @@ size_t Engine::budget() const {
     return lim;
 }
 
-k::Tile Engine::pick_tile(int64_t mb, int64_t nb, int planes) const {
-    if (tune_.gemm_tile == 256) return k::Tile::T256;
-    if (tune_.gemm_tile == 128) return k::Tile::T128;
-    if (tune_.gemm_tile == 64) return k::Tile::T64;
+k::Tile Engine::pick_tile(int64_t mb, int64_t nb, int64_t kp, int planes) const {
+    k::Tile forced;
+    if (k::tile_from_code(tune_.gemm_tile, forced)) return forced;
     // never pad a small tile up to a bigger macro tile
     const int64_t lim = std::min(mb, nb);
     if (lim <= 64) return k::Tile::T64;
-    // largest tile that still gives every CU two workgroups
-    for (int64_t T : {256, 128}) {
-        if (T == 256 && lim <= 128) continue;
-        const int64_t blocks = ((mb + T - 1) / T) * ((nb + T - 1) / T) * planes;
-        if (blocks >= 512) return T == 256 ? k::Tile::T256 : k::Tile::T128;
-    }
-    const int64_t b128 = ((mb + 127) / 128) * ((nb + 127) / 128) * planes;
-    return b128 >= 256 ? k::Tile::T128 : k::Tile::T64;
+    auto blocks = [&](int64_t tm, int64_t tn) { return ((mb + tm - 1) / tm) * ((nb + tn - 1) / tn) * planes; };
+    // k loops of at most 1024: 256 x 128 tiles, two workgroups per CU (measured
+    // ahead of both square tiles there, behind 256 x 256 for long k)
+    if (kp <= 1024 && mb > 128 && nb > 64 && blocks(256, 128) >= 512) return k::Tile::T256x128;
+    // largest square tile that still gives every CU two workgroups
+    if (lim > 128 && blocks(256, 256) >= 512) return k::Tile::T256;
+    if (blocks(128, 128) >= 512) return k::Tile::T128;
+    return blocks(128, 128) >= 256 ? k::Tile::T128 : k::Tile::T64;
 }
 
 Engine::Layout Engine::layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch,
@@ -172,10 +172,12 @@ Engine::Layout Engine::layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int
     L.g = cplx ? 3 : 1;
     L.planes = L.g * s;
     L.kch = kch;
-    L.T = k::tile_rows(pick_tile(mb, nb, L.planes));
     L.kp = round_up(k, kKStep);
-    L.mpad = round_up(mb, L.T);
-    L.npad = round_up(nb, L.T);
+    L.tile = pick_tile(mb, nb, std::min<int64_t>(L.kp, kMaxKChunk), L.planes);
+    L.TM = k::tile_m(L.tile);
+    L.TN = k::tile_n(L.tile);
+    L.mpad = round_up(mb, L.TM);
+    L.npad = round_up(nb, L.TN);
     L.planeA = L.mpad * L.kp;
     L.planeB = L.npad * L.kp;
     L.planeR = L.npad * L.mpad;
@@ -280,7 +282,7 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
     choose_tiling(ta, tb, m, n, k, s, cplx, kch);
     const int64_t mb = tiling_.mb, nb = tiling_.nb;
     const Layout L = layout(ta, tb, m, n, k, s, cplx, kch, mb, nb);
-    tile_ = pick_tile(mb, nb, L.planes);
+    tile_ = L.tile;
     ensure_workspace(L.total);
 
     int32_t* D = reinterpret_cast<int32_t*>(ws_ + L.oD);
@@ -305,7 +307,7 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
         sb.X = sb.kcontig ? B + cw * j0 * ldb : B + cw * j0;
         sb.ldx = ldb;
         sb.rows = nbj;
-        sb.rows_pad = round_up(nbj, L.T);
+        sb.rows_pad = round_up(nbj, L.TN);
         sb.k = k;
         sb.kp = kp;
         sb.ex = E + j0;
@@ -327,7 +329,7 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
                 sa.X = sa.kcontig ? A + cw * i0 * lda : A + cw * i0;
                 sa.ldx = lda;
                 sa.rows = mbi;
-                sa.rows_pad = round_up(mbi, L.T);
+                sa.rows_pad = round_up(mbi, L.TM);
                 sa.k = k;
                 sa.kp = kp;
                 sa.ex = D + i0;
@@ -350,8 +352,8 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
             ga.plane_b = L.planeB;
             ga.plane_r = L.planeR;
             ga.ldr = L.mpad;
-            ga.tiles_m = static_cast<int>((mbi + L.T - 1) / L.T);
-            ga.tiles_n = static_cast<int>((nbj + L.T - 1) / L.T);
+            ga.tiles_m = static_cast<int>((mbi + L.TM - 1) / L.TM);
+            ga.tiles_n = static_cast<int>((nbj + L.TN - 1) / L.TN);
             ga.nplanes = L.planes;
             ga.s = s;
             ga.tab = tab;

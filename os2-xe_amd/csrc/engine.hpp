@@ -33,7 +33,7 @@ struct Tuning {
     bool batched_gemm = true;  // one launch for all planes (the only form here)
     bool pad_lda = true;       // accepted for API parity; plane rows are padded to the GEMM tile anyway
     bool pow2_tiles = true;    // include power-of-two tile candidates
-    int gemm_tile = -1;        // -1 auto, else 256 / 128 / 64
+    int gemm_tile = -1;        // -1 auto, else a tile code: 256 / 128 / 64 (square) or 256128 (256 x 128)
 };
 
 enum class Op { N = 0, T = 1, C = 2 };
@@ -74,7 +74,7 @@ public:
     // shared with batch plans
     const TabSet* device_tables() const { return dtabs_; }
     const Plan& plan_for(int s, int64_t k);
-    k::Tile pick_tile(int64_t mb, int64_t nb, int planes) const;
+    k::Tile pick_tile(int64_t mb, int64_t nb, int64_t kp, int planes) const;
     size_t budget() const;
     int compute_units() const { return cus_; }
 

@@ -20,11 +20,14 @@
 //
 // Kernel shape: macro tile BM x BN, 256 threads = 4 waves in 2 x 2, each wave
 // (BM/2) x (BN/2) from v_mfma_i32_32x32x32_i8 blocks.  k advances 64 bytes per
-// stage; NST = 4 stages are in flight (global -> LDS by DMA, no staging
+// stage; NS stages are in flight (global -> LDS by DMA, no staging
 // registers) and the operand fragments are double-buffered in registers, so
 // each half-iteration issues, in a fixed order, the MFMAs of the current
-// stage, the ds_reads of the next stage and the DMAs of stage + 3; a wave's
-// VMEM and LDS issue never sits in front of a run of its MFMAs.
+// stage, the ds_reads of the next stage and the DMAs of stage + NS - 1; a
+// wave's VMEM and LDS issue never sits in front of a run of its MFMAs.
+// 256 x 256 tiles run one workgroup per CU with NS = 4 (128 KB of LDS);
+// 256 x 128 tiles run two per CU with NS = 3 (72 KB each): with short k loops
+// the residue epilogue of one workgroup then overlaps the MFMAs of the other.
 //
 // MFMA 32x32x32 i8 operand layout (checked on gfx950): lane l holds row
 // (l mod 32) of A / column (l mod 32) of B with 16 k-bytes at k = 16*(l/32);
@@ -47,7 +50,6 @@ typedef __attribute__((address_space(1))) void* glb_ptr;
 namespace {
 
 constexpr int KT = 64;   // k bytes per LDS stage = one plane block column
-constexpr int NST = 4;   // LDS stages
 
 // s_waitcnt immediate waiting for vmcnt <= n only (gfx9 encoding: vmcnt in bits
 // [3:0] and [15:14]; expcnt [6:4] and lgkmcnt [11:8] left at their maximum)
@@ -129,24 +131,24 @@ __device__ __forceinline__ void tile_coords(const GemmArgs& p, uint32_t bid, uin
     tn = in / gs;
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int NS>
 struct TileShape {
     static constexpr int STAGE = (BM + BN) * KT;  // bytes
-    static constexpr int SMEM = NST * STAGE;
+    static constexpr int SMEM = NS * STAGE;
 };
 
-// one BM x BN macro tile of one plane: k loop and residue epilogue; smem holds NST stages
-template <int BM, int BN>
+// one BM x BN macro tile of one plane: k loop and residue epilogue; smem holds NS stages
+template <int BM, int BN, int NS>
 __device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uint32_t tm, uint32_t tn, v4i* smem) {
     constexpr int WTM = BM / 2, WTN = BN / 2;
     constexpr int FM = WTM / 32, FN = WTN / 32;
     constexpr int LA = BM / 64, LB = BN / 64;  // 1 KB DMAs per wave per stage
-    constexpr int STAGE = TileShape<BM, BN>::STAGE;
+    constexpr int STAGE = TileShape<BM, BN, NS>::STAGE;
     constexpr int NR = 2 * (FM + FN);           // fragment reads per stage
     constexpr int NMF = 2 * FM * FN;            // MFMAs per stage
     constexpr int ND = LA + LB;                 // DMAs per stage
-    constexpr int VM_AHEAD = (NST - 3) * ND;    // DMAs still in flight when a stage must have landed
-    static_assert(WTM * WTN * 4 <= NST * STAGE, "epilogue staging must fit in the stage buffers");
+    constexpr int VM_AHEAD = (NS - 3) * ND;    // DMAs still in flight when a stage must have landed
+    static_assert(WTM * WTN * 4 <= NS * STAGE, "epilogue staging must fit in the stage buffers");
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -184,7 +186,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uin
     // buffer that is never read again, keeping the vmcnt arithmetic uniform)
 #define OZ2_DMA(KT_, D_)                                                                                         \
     {                                                                                                            \
-        char* st_ = sbase + ((KT_) % NST) * STAGE;                                                               \
+        char* st_ = sbase + ((KT_) % NS) * STAGE;                                                               \
         const int64_t ko_ = static_cast<int64_t>(min((KT_), nkt - 1)) * 1024;                                    \
         if ((D_) < LA)                                                                                           \
             __builtin_amdgcn_global_load_lds((glb_ptr)(Ag + (D_) * bstride + ko_),                               \
@@ -195,7 +197,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uin
     }
 #define OZ2_READ(NA, NB, KT_, R_)                                                                                \
     {                                                                                                            \
-        const char* s_ = sbase + ((KT_) % NST) * STAGE;                                                          \
+        const char* s_ = sbase + ((KT_) % NS) * STAGE;                                                          \
         const int ks_ = (R_) / (FM + FN), w_ = (R_) % (FM + FN);                                                 \
         if (w_ < FM) NA[ks_][w_] = *reinterpret_cast<const v4i*>(s_ + offa[ks_] + w_ * 2048);                   \
         else NB[ks_][w_ - FM] = *reinterpret_cast<const v4i*>(s_ + offb[ks_] + (w_ - FM) * 2048);               \
@@ -227,21 +229,21 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uin
         __builtin_amdgcn_sched_barrier(0);                              \
     }
 
-    // prologue: stages 0 .. NST-2 in flight, stage 0 landed and read
+    // prologue: stages 0 .. NS-2 in flight, stage 0 landed and read
 #pragma unroll
-    for (int s0 = 0; s0 < NST - 1; ++s0)
+    for (int s0 = 0; s0 < NS - 1; ++s0)
 #pragma unroll
         for (int d = 0; d < ND; ++d) OZ2_DMA(s0, d)
-    __builtin_amdgcn_s_waitcnt(vm_wait((NST - 2) * ND));
+    __builtin_amdgcn_s_waitcnt(vm_wait((NS - 2) * ND));
     __builtin_amdgcn_s_barrier();
 #pragma unroll
     for (int r = 0; r < NR; ++r) OZ2_READ(fa0, fb0, 0, r)
 
     for (int kt = 0; kt < nkt; kt += 2) {
         OZ2_WAIT()
-        OZ2_HALF(fa0, fb0, fa1, fb1, kt + 1, kt + NST - 1)
+        OZ2_HALF(fa0, fb0, fa1, fb1, kt + 1, kt + NS - 1)
         OZ2_WAIT()
-        OZ2_HALF(fa1, fb1, fa0, fb0, kt + 2, kt + NST)  // the last pass reads a stage that is not used
+        OZ2_HALF(fa1, fb1, fa0, fb0, kt + 2, kt + NS)  // the last pass reads a stage that is not used
     }
 #undef OZ2_DMA
 #undef OZ2_READ
@@ -253,23 +255,23 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uin
                              static_cast<int64_t>(tn) * BN + wn * WTN, sbase + wave * (WTM * WTN), lane);
 }
 
-template <int BM, int BN, int MINB>
+template <int BM, int BN, int MINB, int NS>
 __global__ void __launch_bounds__(256, MINB) igemm_residue_kernel(GemmArgs p) {
-    __shared__ v4i smem[TileShape<BM, BN>::SMEM / 16];
+    __shared__ v4i smem[TileShape<BM, BN, NS>::SMEM / 16];
     uint32_t plane, tm, tn;
     tile_coords(p, xcd_remap(blockIdx.x, gridDim.x), plane, tm, tn);
-    gemm_tile<BM, BN>(p, plane, tm, tn, smem);
+    gemm_tile<BM, BN, NS>(p, plane, tm, tn, smem);
 }
 
 // Grouped forms over many problems; owner[w] is the item (problem, k-chunk) of
 // work index w and start[] its first index, items listed largest first.
 // Persistent: a fixed grid takes work indices from a counter until all are done.
-template <int BM, int BN, int MINB>
+template <int BM, int BN, int MINB, int NS>
 __global__ void __launch_bounds__(256, MINB) igemm_grouped_kernel(const GemmArgs* __restrict__ items,
                                                                   const uint32_t* __restrict__ start,
                                                                   const uint32_t* __restrict__ owner,
                                                                   uint32_t total, uint32_t* __restrict__ counter) {
-    __shared__ v4i smem[TileShape<BM, BN>::SMEM / 16];
+    __shared__ v4i smem[TileShape<BM, BN, NS>::SMEM / 16];
     __shared__ uint32_t next;
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
@@ -281,22 +283,22 @@ __global__ void __launch_bounds__(256, MINB) igemm_grouped_kernel(const GemmArgs
         const GemmArgs& p = items[it];
         uint32_t plane, tm, tn;
         tile_coords(p, w - start[it], plane, tm, tn);
-        gemm_tile<BM, BN>(p, plane, tm, tn, smem);
+        gemm_tile<BM, BN, NS>(p, plane, tm, tn, smem);
     }
 }
 
 // One workgroup per tile, work indices spread over the XCDs in contiguous ranges.
-template <int BM, int BN, int MINB>
+template <int BM, int BN, int MINB, int NS>
 __global__ void __launch_bounds__(256, MINB) igemm_grouped_blocks_kernel(const GemmArgs* __restrict__ items,
                                                                          const uint32_t* __restrict__ start,
                                                                          const uint32_t* __restrict__ owner) {
-    __shared__ v4i smem[TileShape<BM, BN>::SMEM / 16];
+    __shared__ v4i smem[TileShape<BM, BN, NS>::SMEM / 16];
     const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t it = owner[w];
     const GemmArgs& p = items[it];
     uint32_t plane, tm, tn;
     tile_coords(p, w - start[it], plane, tm, tn);
-    gemm_tile<BM, BN>(p, plane, tm, tn, smem);
+    gemm_tile<BM, BN, NS>(p, plane, tm, tn, smem);
 }
 
 __global__ void reset_counters(uint32_t* c, int n) {
@@ -305,13 +307,24 @@ __global__ void reset_counters(uint32_t* c, int n) {
 
 }  // namespace
 
-int tile_rows(Tile t) { return t == Tile::T256 ? 256 : t == Tile::T128 ? 128 : 64; }
+int tile_m(Tile t) { return t == Tile::T256 || t == Tile::T256x128 ? 256 : t == Tile::T128 ? 128 : 64; }
+int tile_n(Tile t) { return t == Tile::T256 ? 256 : t == Tile::T64 ? 64 : 128; }
+int tile_code(Tile t) { return t == Tile::T256x128 ? 256128 : tile_m(t); }
+bool tile_from_code(int code, Tile& t) {
+    switch (code) {
+        case 256: t = Tile::T256; return true;
+        case 128: t = Tile::T128; return true;
+        case 64: t = Tile::T64; return true;
+        case 256128: t = Tile::T256x128; return true;
+        default: return false;
+    }
+}
 
-// blocks of the persistent grouped GEMM per CU (LDS: 128 / 64 / 32 KB per block)
-static int tile_blocks_per_cu(Tile t) { return t == Tile::T256 ? 1 : t == Tile::T128 ? 2 : 4; }
+// workgroups per CU (LDS: 128 / 64 / 32 / 72 KB per workgroup)
+static int tile_blocks_per_cu(Tile t) { return t == Tile::T256 ? 1 : t == Tile::T64 ? 4 : 2; }
 
 void prep_gemm(std::vector<GroupedLaunch>& out, const std::vector<GemmJob>& jobs) {
-    GroupedLaunch by[3];
+    GroupedLaunch by[4];
     for (const GemmJob& j : jobs) {
         const uint32_t nblk = static_cast<uint32_t>(j.args.tiles_m) * j.args.tiles_n * j.args.nplanes;
         if (nblk == 0 || j.args.klen <= 0) continue;
@@ -331,24 +344,26 @@ void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* count
     if (persistent) {
         hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, counter, 1);
         const uint32_t grid = std::min<uint32_t>(g.blocks, static_cast<uint32_t>(cus * tile_blocks_per_cu(t)));
-#define OZ2_LAUNCH(BM, MINB)                                                                                     \
-    hipLaunchKernelGGL((igemm_grouped_kernel<BM, BM, MINB>), dim3(grid), dim3(256), 0, st, items, g.d_start,    \
+#define OZ2_LAUNCH(BM, BN, MINB, NS)                                                                             \
+    hipLaunchKernelGGL((igemm_grouped_kernel<BM, BN, MINB, NS>), dim3(grid), dim3(256), 0, st, items, g.d_start, \
                        g.d_owner, g.blocks, counter)
         switch (t) {
-            case Tile::T256: OZ2_LAUNCH(256, 1); break;
-            case Tile::T128: OZ2_LAUNCH(128, 2); break;
-            case Tile::T64: OZ2_LAUNCH(64, 4); break;
+            case Tile::T256: OZ2_LAUNCH(256, 256, 1, 4); break;
+            case Tile::T128: OZ2_LAUNCH(128, 128, 2, 4); break;
+            case Tile::T64: OZ2_LAUNCH(64, 64, 4, 4); break;
+            case Tile::T256x128: OZ2_LAUNCH(256, 128, 2, 3); break;
         }
 #undef OZ2_LAUNCH
         return;
     }
-#define OZ2_LAUNCH(BM, MINB)                                                                                     \
-    hipLaunchKernelGGL((igemm_grouped_blocks_kernel<BM, BM, MINB>), dim3(g.blocks), dim3(256), 0, st, items,     \
+#define OZ2_LAUNCH(BM, BN, MINB, NS)                                                                             \
+    hipLaunchKernelGGL((igemm_grouped_blocks_kernel<BM, BN, MINB, NS>), dim3(g.blocks), dim3(256), 0, st, items, \
                        g.d_start, g.d_owner)
     switch (t) {
-        case Tile::T256: OZ2_LAUNCH(256, 1); break;
-        case Tile::T128: OZ2_LAUNCH(128, 2); break;
-        case Tile::T64: OZ2_LAUNCH(64, 4); break;
+        case Tile::T256: OZ2_LAUNCH(256, 256, 1, 4); break;
+        case Tile::T128: OZ2_LAUNCH(128, 128, 2, 4); break;
+        case Tile::T64: OZ2_LAUNCH(64, 64, 4, 4); break;
+        case Tile::T256x128: OZ2_LAUNCH(256, 128, 2, 3); break;
     }
 #undef OZ2_LAUNCH
 }
@@ -356,11 +371,15 @@ void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* count
 void igemm_residue(hipStream_t st, Tile tile, const GemmArgs& a) {
     const uint32_t nblk = static_cast<uint32_t>(a.tiles_m) * a.tiles_n * a.nplanes;
     if (nblk == 0 || a.klen <= 0) return;
+#define OZ2_LAUNCH(BM, BN, MINB, NS) \
+    hipLaunchKernelGGL((igemm_residue_kernel<BM, BN, MINB, NS>), dim3(nblk), dim3(256), 0, st, a)
     switch (tile) {
-        case Tile::T256: hipLaunchKernelGGL((igemm_residue_kernel<256, 256, 1>), dim3(nblk), dim3(256), 0, st, a); break;
-        case Tile::T128: hipLaunchKernelGGL((igemm_residue_kernel<128, 128, 2>), dim3(nblk), dim3(256), 0, st, a); break;
-        case Tile::T64: hipLaunchKernelGGL((igemm_residue_kernel<64, 64, 4>), dim3(nblk), dim3(256), 0, st, a); break;
+        case Tile::T256: OZ2_LAUNCH(256, 256, 1, 4); break;
+        case Tile::T128: OZ2_LAUNCH(128, 128, 2, 4); break;
+        case Tile::T64: OZ2_LAUNCH(64, 64, 4, 4); break;
+        case Tile::T256x128: OZ2_LAUNCH(256, 128, 2, 3); break;
     }
+#undef OZ2_LAUNCH
 }
 
 }  // namespace k

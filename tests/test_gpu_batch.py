@@ -93,7 +93,7 @@ def test_batch_matches_single_calls(oz, graph):
 def test_forced_tiles_identical(oz):
     ps = build(PROBLEMS[:8], seed=11)
     ref = [reference(oz, p) for p in ps]
-    for t in (256, 128, 64):
+    for t in (256, 128, 64, 256128):
         log(f"batch with every macro tile {t}")
         Cs = [to_dev(p["C0"].copy()) for p in ps]
         b = oz2.Batch(oz, [desc(p, C) for p, C in zip(ps, Cs)], tiles=[t] * len(ps), graph=False)
@@ -158,6 +158,9 @@ def test_batch_argument_errors(oz):
     d[0].update(k=140000, ldb=140000, num_moduli=2)  # s too small for k: make_plan's error (hpp:186), before any launch
     with pytest.raises(oz2.OZ2InvalidArgument):
         oz2.Batch(oz, d)
+    d = [desc(p, C) for p, C in zip(ps, Cs)]
+    with pytest.raises(oz2.OZ2InvalidArgument, match="tile"):
+        oz2.Batch(oz, d, tiles=[256, 96])
 
 
 @pytest.mark.parametrize("persistent,largest_first", [(True, True), (False, False), (True, False)])
@@ -181,7 +184,7 @@ def test_autotuned_plan(oz):
     d = [desc(p, C) for p, C in zip(ps, Cs)]
     tune = oz.batch_autotune(d, bits=53)
     log(f"autotune: {tune}")
-    assert all(t in (0, 64, 128, 256) for t in tune["tiles"]) and tune["ms"] > 0
+    assert all(t in (0, 64, 128, 256, 256128) for t in tune["tiles"]) and tune["ms"] > 0
     for p, C in zip(ps, Cs):
         C.copy_(to_dev(p["C0"]))
     b = oz2.Batch(oz, d, graph=True, tiles=tune["tiles"], persistent=tune["persistent"],

@@ -153,7 +153,7 @@ def test_stage_times_and_tiling(oz):
     assert ti.workspace_bytes > 3 * 16 * 512 * 512
 
 
-@pytest.mark.parametrize("tile", [256, 128, 64])
+@pytest.mark.parametrize("tile", [256, 128, 64, 256128])
 def test_every_gemm_tile_bit_identical(oz, tile):
     oz.set_tuning(gemm_tile=tile)
     try:
@@ -187,6 +187,13 @@ def test_large_cases(oz):
     _sampled_large(oz, "N", "N", 2048, 2048, 4096, 16, "sincos")
     _sampled_large(oz, "N", "N", 3000, 2500, 4000, 8, "phi")
     _sampled_large(oz, "T", "N", 2500, 3000, 4000, 20, "phi")
+
+
+def test_short_k_rectangular_tile(oz):
+    # k loops up to 1024 with enough workgroups take the 256 x 128 macro tile (two per CU);
+    # ragged m and n against the tile, sampled entries against the exact closed form
+    _sampled_large(oz, "N", "T", 1500, 1300, 500, 16, "phi")
+    _sampled_large(oz, "T", "N", 1100, 2100, 1000, 14, "sincos")
 
 
 def test_stream_and_events(oz):
