@@ -67,8 +67,8 @@ BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector
         q.plan = make_plan(s_[i], g.k);  // throws invalid_argument when s is too small for k (hpp:186)
         const int planes = s_[i];
         q.kp = round_up(g.k, kKStep);
-        q.kch = (q.kp + kMaxKChunk - 1) / kMaxKChunk;
-        if (tiles.empty() || tiles[i] <= 0) q.tile = eng_.pick_tile(g.m, g.n, std::min(q.kp, kMaxKChunk), planes);
+        q.kch = k_chunks(q.kp);
+        if (tiles.empty() || tiles[i] <= 0) q.tile = eng_.pick_tile(g.m, g.n, k_chunk_len(q.kp), planes);
         else k::tile_from_code(tiles[i], q.tile);
         tile_codes_[i] = k::tile_code(q.tile);
         q.mpad = round_up(g.m, k::tile_m(q.tile));
@@ -167,8 +167,8 @@ BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector
                 a.plane_b = q.planeB;
                 a.plane_r = q.planeR;
                 a.ldr = q.mpad;
-                a.k0 = c * kMaxKChunk;
-                a.klen = std::min(kMaxKChunk, q.kp - a.k0);
+                a.k0 = c * k_chunk_len(q.kp);
+                a.klen = std::min(k_chunk_len(q.kp), q.kp - a.k0);
                 a.R = reinterpret_cast<uint8_t*>(base + q.oR) + c * s_[i] * q.planeR;
                 a.tiles_m = static_cast<int>(q.mpad / k::tile_m(q.tile));
                 a.tiles_n = static_cast<int>(q.npad / k::tile_n(q.tile));

@@ -173,7 +173,7 @@ Engine::Layout Engine::layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int
     L.planes = L.g * s;
     L.kch = kch;
     L.kp = round_up(k, kKStep);
-    L.tile = pick_tile(mb, nb, std::min<int64_t>(L.kp, kMaxKChunk), L.planes);
+    L.tile = pick_tile(mb, nb, k_chunk_len(L.kp), L.planes);
     L.TM = k::tile_m(L.tile);
     L.TN = k::tile_n(L.tile);
     L.mpad = round_up(mb, L.TM);
@@ -199,7 +199,7 @@ Engine::Layout Engine::layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int
 }
 
 size_t Engine::workspace_for(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx) {
-    const int64_t kch = (round_up(k, kKStep) + kMaxKChunk - 1) / kMaxKChunk;
+    const int64_t kch = k_chunks(round_up(k, kKStep));
     choose_tiling(ta, tb, m, n, k, s, cplx, kch);
     return tiling_.workspace_bytes;
 }
@@ -278,7 +278,8 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
     const TabSet* tab = dtabs_ + s;
 
     const int64_t kp = round_up(k, kKStep);
-    const int64_t kch = (kp + kMaxKChunk - 1) / kMaxKChunk;
+    const int64_t kch = k_chunks(kp);
+    const int64_t klen = k_chunk_len(kp);
     choose_tiling(ta, tb, m, n, k, s, cplx, kch);
     const int64_t mb = tiling_.mb, nb = tiling_.nb;
     const Layout L = layout(ta, tb, m, n, k, s, cplx, kch, mb, nb);
@@ -358,8 +359,8 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
             ga.s = s;
             ga.tab = tab;
             for (int64_t c = 0; c < kch; ++c) {
-                ga.k0 = c * kMaxKChunk;
-                ga.klen = std::min(kMaxKChunk, kp - ga.k0);
+                ga.k0 = c * klen;
+                ga.klen = std::min(klen, kp - ga.k0);
                 ga.R = R + c * L.planes * L.planeR;
                 k::igemm_residue(stream_, tile_, ga);
             }

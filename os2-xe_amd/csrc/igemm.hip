@@ -3,8 +3,8 @@
 //
 // Replaces the reference's oneMKL gemm_batch int8 -> int32 calls
 // (src/ozaki2_gpu.hpp:1023-1042 DGEMM, 880-891 ZGEMM): plane t computes
-// Ct = op(A)_t * op(B)_t exactly in int32 (k-chunks <= 130944 keep
-// |sum| <= kc * 2^14 < 2^31).  Instead of storing int32, the epilogue reduces
+// Ct = op(A)_t * op(B)_t exactly in int32 (k-chunks <= 65408 keep
+// |sum| <= kc * 2^14 < 2^30).  Instead of storing int32, the epilogue reduces
 // every entry mod m_t to a byte, the only information the CRT needs
 // (hpp:590-601), cutting the CRT's reads from 4s to s bytes per entry.
 //
@@ -73,7 +73,7 @@ __device__ __forceinline__ void residue_epilogue(const GemmArgs& p, v16i (&acc)[
                                                  int64_t row0, int64_t col0, char* wlds, int lane) {
     constexpr int WTM = FM * 32, WTN = FN * 32, NCH = WTM / 16;
     const int t = static_cast<int>(plane % static_cast<uint32_t>(p.s));
-    const TabSet* __restrict__ tb = p.tab;
+    const auto tb = const_tab(p.tab);
     const uint32_t m = tb->m[t];
     const int fr = lane & 31, fc = lane >> 5;
     auto lds_at = [&](int c, int r) { return wlds + c * WTM + ((((r >> 4) ^ (c & (NCH - 1))) << 4) | (r & 15)); };
@@ -90,7 +90,7 @@ __device__ __forceinline__ void residue_epilogue(const GemmArgs& p, v16i (&acc)[
                     *reinterpret_cast<uint32_t*>(lds_at(j * 32 + fr, i * 32 + 8 * q + 4 * fc)) = w;
                 }
     } else {
-        const uint32_t mu = tb->mu[t], c16 = tb->c16[t], c31 = tb->c31[t];
+        const uint32_t mu = tb->mu[t], c16 = tb->c16[t], c30 = tb->c30[t];
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -99,7 +99,7 @@ __device__ __forceinline__ void residue_epilogue(const GemmArgs& p, v16i (&acc)[
                 for (int q = 0; q < 4; ++q) {
                     uint32_t w = 0;
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) w |= residue_m(static_cast<uint32_t>(acc[i][j][4 * q + b]), m, mu, c16, c31) << (8 * b);
+                    for (int b = 0; b < 4; ++b) w |= residue_m30(static_cast<uint32_t>(acc[i][j][4 * q + b]), m, mu, c16, c30) << (8 * b);
                     *reinterpret_cast<uint32_t*>(lds_at(j * 32 + fr, i * 32 + 8 * q + 4 * fc)) = w;
                 }
     }

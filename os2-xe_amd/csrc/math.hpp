@@ -45,8 +45,17 @@ OZ2_HD uint32_t mad_i24(int32_t a, int32_t b, uint32_t c) {
     return static_cast<uint32_t>(static_cast<uint64_t>(sa * sb)) + c;
 #endif
 }
+#if defined(__HIP_DEVICE_COMPILE__)
+// the LLVM intrinsic behind v_mul_hi_u32_u24 (no clang builtin names it); called directly,
+// it takes the low 24 bits of each operand itself, so no masking instructions are emitted
+extern "C" __device__ uint32_t oz2_llvm_mulhi_u24(uint32_t, uint32_t) __asm("llvm.amdgcn.mulhi.u24");
+#endif
 OZ2_HD uint32_t mulhi24(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return oz2_llvm_mulhi_u24(a, b);
+#else
     return static_cast<uint32_t>((static_cast<uint64_t>(a & 0xFFFFFFu) * static_cast<uint64_t>(b & 0xFFFFFFu)) >> 32);
+#endif
 }
 OZ2_HD uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
@@ -68,22 +77,6 @@ OZ2_HD uint32_t udot4(uint32_t a, uint32_t b, uint32_t c) {
     for (int i = 0; i < 32; i += 8) c += ((a >> i) & 0xFFu) * ((b >> i) & 0xFFu);
     return c;
 #endif
-}
-
-// x mod m for x < 2^24, 2 <= m <= 255, mu = floor(2^31 / m) (< 2^24 since m >= 129).
-// Barrett with q = floor(x mu / 2^31): q is floor(x/m) or one less, so r = x - q m
-// lies in [0, 2m) and one conditional subtraction finishes.
-OZ2_HD uint32_t mod24(uint32_t x, uint32_t m, uint32_t mu) {
-    const uint64_t prod = (static_cast<uint64_t>(mulhi24(x, mu)) << 32) | mul24(x, mu);  // 48-bit x*mu
-    const uint32_t q = static_cast<uint32_t>(prod >> 31);  // < 2^17
-    const uint32_t r = mad_i24(static_cast<int32_t>(q), -static_cast<int32_t>(m), x);
-    return umin(r, r - m);
-}
-
-// x mod m for any 32-bit x (m <= 255): fold the high half with c16 = 2^16 mod m,
-// (x >> 16) c16 + (x & 0xffff) <= 65535 * 254 + 65535 < 2^24, then Barrett.
-OZ2_HD uint32_t mod32(uint32_t x, uint32_t m, uint32_t mu, uint32_t c16) {
-    return mod24(mul24(x >> 16, c16) + (x & 0xFFFFu), m, mu);
 }
 
 OZ2_HD int clz32(uint32_t x) { return x ? __builtin_clz(x) : 32; }
@@ -206,18 +199,25 @@ OZ2_HD int32_t sym_value(const uint32_t (&a)[4], Tab tb, int t) {
     return static_cast<int32_t>(limb_residue<F>(a, tb, t)) - static_cast<int32_t>(tb->h[t]);
 }
 
-// GEMM epilogue: int32 dot product x (as bits) -> x mod m in [0, m), m <= 255.
-// c31 = m ceil(2^31 / m) lies in [2^31, 2^31 + m), so x + c31 is x shifted into
-// [0, 2^32) without changing its residue for every x < 2^31 - 255; the GEMM's
-// accumulators satisfy |x| <= 130944 * 2^14 < 2^31 - 2^21.
-OZ2_HD uint32_t residue_m(uint32_t x, uint32_t m, uint32_t mu, uint32_t c16, uint32_t c31) {
-    return mod32(x + c31, m, mu, c16);
+// GEMM epilogue: int32 dot product x (as bits) of one k-chunk -> x mod m in [0, m),
+// m <= 255.  A chunk holds at most kMaxKChunk = 65408 k, so |x| <= 65408 * 2^14 <
+// 2^30 - 255, and with c30 = m ceil(2^30 / m) in [2^30, 2^30 + m) the value u = x + c30
+// lies in [0, 2^31) and keeps x's residue.  One v_mad_i32_i24 folds the high half:
+// y = u + (u >> 16)(c16 - 2^16) = (u >> 16) c16 + (u & 0xffff) <= (2^15 - 1) 254 + 65535
+// < 2^23, so 2y < 2^24 and q = mulhi24(2y, mu) = floor(y mu / 2^31) is floor(y/m) or
+// one less; r = y - q m lies in [0, 2m).
+OZ2_HD uint32_t residue_m30(uint32_t x, uint32_t m, uint32_t mu, uint32_t c16, uint32_t c30) {
+    const uint32_t u = x + c30;
+    const uint32_t y = mad_i24(static_cast<int32_t>(u >> 16), static_cast<int32_t>(c16) - 65536, u);
+    const uint32_t q = mulhi24(y + y, mu);
+    const uint32_t r = mad_i24(static_cast<int32_t>(q), -static_cast<int32_t>(m), y);
+    return umin(r, r - m);
 }
 template <typename Tab>
 OZ2_HD uint32_t acc_residue(uint32_t x, Tab tb, int t) {
     const uint32_t m = tb->m[t];
     if (m == 256) return x & 0xFFu;
-    return residue_m(x, m, tb->mu[t], tb->c16[t], tb->c31[t]);
+    return residue_m30(x, m, tb->mu[t], tb->c16[t], tb->c30[t]);
 }
 
 OZ2_HD double ldexp_(double v, int e) {

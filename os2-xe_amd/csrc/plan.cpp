@@ -163,7 +163,7 @@ void make_tabsets(TabSet* out) {
             tb.off[0][t] = (m - pow2_mod(62, m) + h) % m;
             tb.off[1][t] = (m - pow2_mod(104, m) + h) % m;
             for (int j = 0; j < 16; ++j) tb.c8[t][j >> 2] |= static_cast<uint32_t>(pow2_mod(8 * j, m)) << (8 * (j & 3));
-            tb.c31[t] = m == 256 ? 0 : m * static_cast<uint32_t>(((uint64_t{1} << 31) + m - 1) / m);
+            tb.c30[t] = m == 256 ? 0 : m * static_cast<uint32_t>(((uint64_t{1} << 30) + m - 1) / m);
             tb.frac[t] = static_cast<float>(static_cast<double>(p.y[t]) / m);
             for (int l = 0; l < tb.L16; ++l) tb.W16[t][l] = (p.W[t].word32(l / 2) >> (16 * (l % 2))) & 0xFFFFu;
         }

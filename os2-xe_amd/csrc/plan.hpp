@@ -17,13 +17,24 @@ inline constexpr int kMaxLimbs = 6;           // 32-bit words of M for s = 24
 inline constexpr int kMaxScaleBits = 100;     // tk cap (hpp:29)
 inline constexpr int kNarrowScaleBits = 62;   // tk <= 62: |x| < 2^63 (hpp:30)
 inline constexpr int32_t kNonFinite = INT32_MIN;
-// Largest k-chunk whose int32 accumulation of int8 products is exact:
-// |sum| <= kc * 128 * 128 < 2^31.  The reference uses 131071 (hpp:32); the GEMM
-// here steps k in multiples of 128, so its chunk is the largest multiple of 128
-// below that bound.
+// k-chunks of the INT8 GEMM.  The reference splits k at 131071 (hpp:32), the
+// largest chunk whose int32 accumulation of int8 products is exact
+// (|sum| <= kc * 2^14 < 2^31).  Here a chunk holds at most 65408 k (a multiple of
+// the 128-byte k-step with |sum| < 2^30), which lets the GEMM's residue epilogue use
+// its shorter reduction, and the chunks of one product are of equal length.
 inline constexpr int64_t kRefMaxKChunk = 131071;
 inline constexpr int64_t kKStep = 128;
-inline constexpr int64_t kMaxKChunk = (kRefMaxKChunk / kKStep) * kKStep;  // 130944
+inline constexpr int64_t kMaxKChunk = 65408;
+// chunk length for kp (a multiple of kKStep): kp split into ceil(kp / kMaxKChunk)
+// near-equal parts, rounded up to the k-step (<= kMaxKChunk, which is a multiple of it)
+inline constexpr int64_t k_chunk_len(int64_t kp) {
+    const int64_t parts = kp > 0 ? (kp + kMaxKChunk - 1) / kMaxKChunk : 1;
+    return ((kp + parts - 1) / parts + kKStep - 1) / kKStep * kKStep;
+}
+// chunks of that length covering kp, none empty
+inline constexpr int64_t k_chunks(int64_t kp) {
+    return kp > 0 ? (kp + k_chunk_len(kp) - 1) / k_chunk_len(kp) : 1;
+}
 
 // 32-bit words of M per s: ceil-free form of the reference's limbs_for (hpp:55-59)
 constexpr int limbs_for(int s) { return s <= 8 ? 2 : s <= 12 ? 3 : s <= 16 ? 4 : s <= 20 ? 5 : 6; }
@@ -77,7 +88,7 @@ struct TabSet {
     uint32_t c16[kMaxModuli];               // 2^16 mod m
     uint32_t off[2][kMaxModuli];            // (m - 2^OFF mod m + h) mod m, OFF = 62 (narrow), 104 (wide)
     uint32_t c8[kMaxModuli][4];             // bytes 2^(8 j) mod m, j = 0..15, four to a word (low first)
-    uint32_t c31[kMaxModuli];               // m * ceil(2^31 / m) (GEMM residue epilogue offset; 0 for m = 256)
+    uint32_t c30[kMaxModuli];               // m * ceil(2^30 / m) (GEMM residue epilogue offset; 0 for m = 256)
     // CRT (hpp:222-236)
     float frac[kMaxModuli];                 // fp32(y_t / m_t)
     uint32_t W16[kMaxModuli][2 * kMaxLimbs];  // 16-bit limbs of W_t

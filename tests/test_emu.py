@@ -95,9 +95,13 @@ def test_split_complex_vs_oracle(s, tk, conj, rng):
 
 @pytest.mark.parametrize("s", [2, 7, 14, 16, 24])
 def test_acc_residue(s, rng):
-    # documented domain x < 2^31 - 255 (GEMM accumulators: |x| <= 130944 * 2^14)
-    x = rng.integers(-2 ** 31, 2 ** 31 - 255, 100000, dtype=np.int64)
-    x[:8] = [0, -1, 1, -2 ** 31, 2 ** 31 - 256, 130944 * 2 ** 14, -130944 * 2 ** 14, 130944 * 127 * 127]
+    # documented domain |x| < 2^30 - 255 (GEMM accumulators of one k-chunk: |x| <= 65408 * 2^14),
+    # its edges, and the largest fold value y = (2^15 - 1) c16 + 65535 (x + c30 = 2^31 - 1)
+    lim = 2 ** 30 - 256
+    x = rng.integers(-lim, lim + 1, 100000, dtype=np.int64)
+    x[:10] = [0, -1, 1, -lim, lim, 65408 * 2 ** 14, -65408 * 2 ** 14, 65408 * 127 * 127, -65408 * 128 * 127, 2 ** 14]
+    for j, m in enumerate(O.moduli(s)[1:]):
+        x[10 + j] = 2 ** 31 - 1 - m * ((2 ** 30 + m - 1) // m)  # u = 2^31 - 1 for modulus m
     out = np.zeros((s, x.size), dtype=np.uint8)
     xi = x.astype(np.int32)
     _emu.emu_acc_residue(P(xi), x.size, s, P(out))

@@ -86,9 +86,12 @@ def test_workspace_too_small(oz):
 
 
 def test_k_chunks(oz):
-    # test:488-489: k > 131071 splits the int32 accumulation
+    # test:488-489: long k splits the int32 accumulation (here into equal chunks of at most 65408)
     real_case(oz, "N", "N", 16, 24, 300000, 18, "phi", phi=0.5)
+    assert oz.last_tiling().k_chunks == 5
     real_case(oz, "T", "N", 8, 8, 140000, 18, "uniform")
+    assert oz.last_tiling().k_chunks == 3
+    real_case(oz, "N", "T", 40, 36, 65409, 16, "phi")  # one k past a chunk: two chunks of 32768
     assert oz.last_tiling().k_chunks == 2
 
 

@@ -47,7 +47,7 @@ def test_max_magnitude(oz):
 
 def test_k_chunks(oz):
     complex_case(oz, "N", "N", 8, 12, 140000, 18, "uniform")
-    assert oz.last_tiling().k_chunks == 2  # the plan is at 2k; the chunks follow k
+    assert oz.last_tiling().k_chunks == 3  # the plan is at 2k; the chunks follow k
 
 
 def test_tiling(oz):
