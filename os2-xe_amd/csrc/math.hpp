@@ -27,6 +27,15 @@ using ConstTab = const __attribute__((address_space(4))) TabSet*;
 __device__ __forceinline__ ConstTab const_tab(const TabSet* p) { return (ConstTab)p; }
 #endif
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// The LLVM intrinsics behind v_mul_hi_u32_u24 and v_mul_i32_i24 (no clang builtin names
+// them).  Called directly they take the low 24 bits of each operand themselves, so no
+// masking instructions are emitted, and the optimizer cannot turn a 24-bit product whose
+// low bits alone are used into a full 32- or 64-bit multiply (quarter rate).
+extern "C" __device__ uint32_t oz2_llvm_mulhi_u24(uint32_t, uint32_t) __asm("llvm.amdgcn.mulhi.u24");
+extern "C" __device__ int32_t oz2_llvm_mul_i24(int32_t, int32_t) __asm("llvm.amdgcn.mul.i24");
+#endif
+
 // v_mul_u32_u24 / v_mul_hi_u32_u24: low 32 and high 16 bits of a 24 x 24-bit product
 OZ2_HD uint32_t mul24(uint32_t a, uint32_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -38,18 +47,13 @@ OZ2_HD uint32_t mul24(uint32_t a, uint32_t b) {
 // v_mad_i32_i24: a * b + c with a, b sign-extended from 24 bits (low 32 bits of the result)
 OZ2_HD uint32_t mad_i24(int32_t a, int32_t b, uint32_t c) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return static_cast<uint32_t>(__mul24(a, b)) + c;
+    return static_cast<uint32_t>(oz2_llvm_mul_i24(a, b)) + c;
 #else
     const int64_t sa = static_cast<int64_t>(static_cast<uint64_t>(a) << 40) >> 40;
     const int64_t sb = static_cast<int64_t>(static_cast<uint64_t>(b) << 40) >> 40;
     return static_cast<uint32_t>(static_cast<uint64_t>(sa * sb)) + c;
 #endif
 }
-#if defined(__HIP_DEVICE_COMPILE__)
-// the LLVM intrinsic behind v_mul_hi_u32_u24 (no clang builtin names it); called directly,
-// it takes the low 24 bits of each operand itself, so no masking instructions are emitted
-extern "C" __device__ uint32_t oz2_llvm_mulhi_u24(uint32_t, uint32_t) __asm("llvm.amdgcn.mulhi.u24");
-#endif
 OZ2_HD uint32_t mulhi24(uint32_t a, uint32_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return oz2_llvm_mulhi_u24(a, b);
@@ -57,7 +61,6 @@ OZ2_HD uint32_t mulhi24(uint32_t a, uint32_t b) {
     return static_cast<uint32_t>((static_cast<uint64_t>(a & 0xFFFFFFu) * static_cast<uint64_t>(b & 0xFFFFFFu)) >> 32);
 #endif
 }
-OZ2_HD uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
 // v_dot2_u32_u16: c + a.lo * b.lo + a.hi * b.hi (16-bit halves, no clamping)
 OZ2_HD uint32_t udot2(uint32_t a, uint32_t b, uint32_t c) {
@@ -167,10 +170,18 @@ OZ2_HD void to_limbs(uint64_t bits, int e, bool zero, uint32_t (&a)[4]) {
     }
 }
 
+// Exact quotient by a modulus 129 <= m <= 255 for 0 <= v < 2^23: with mu = ceil(2^31 / m)
+// (< 2^24) and e = mu m - 2^31 in [0, m), v mu / 2^31 = v/m + v e / (m 2^31), and
+// v e < 2^23 * 255 < 2^31 keeps the excess below 1/m, so q = mulhi24(2v, mu) =
+// floor(v mu / 2^31) = floor(v/m) and v - q m (one v_mad_i32_i24) is v mod m.
+OZ2_HD uint32_t mod_m23(uint32_t v, uint32_t m, uint32_t mu) {
+    const uint32_t q = mulhi24(v + v, mu);
+    return mad_i24(static_cast<int32_t>(q), -static_cast<int32_t>(m), v);
+}
+
 // (x + h) mod m in [0, m) for modulus index t (m <= 255) from the octets of u = 2^OFF + x:
 // v = (h - 2^OFF mod m) + sum_j byte_j(u) (2^(8 j) mod m), one v_dot4 per word;
-// v <= 14 * 255 * 254 + 254 < 2^20, so q = mulhi24(2v, mu) = floor(v mu / 2^31) is
-// floor(v/m) or one less and r = v - q m lies in [0, 2m).
+// v <= 14 * 255 * 254 + 254 < 2^20.
 template <Limbs F, typename Tab>
 OZ2_HD uint32_t limb_residue(const uint32_t (&a)[4], Tab tb, int t) {
     constexpr bool W = F == Limbs::Octets14;
@@ -180,9 +191,7 @@ OZ2_HD uint32_t limb_residue(const uint32_t (&a)[4], Tab tb, int t) {
         v = udot4(a[2], tb->c8[t][2], v);
         v = udot4(a[3], tb->c8[t][3], v);
     }
-    const uint32_t q = mulhi24(v << 1, tb->mu[t]);  // < 2^13
-    const uint32_t r = mad_i24(static_cast<int32_t>(q), -static_cast<int32_t>(tb->m[t]), v);
-    return umin(r, r - tb->m[t]);
+    return mod_m23(v, tb->m[t], tb->mu[t]);
 }
 
 // int8 byte of the symmetric residue of x for modulus index t (m = 256 only at t = 0)
@@ -204,14 +213,11 @@ OZ2_HD int32_t sym_value(const uint32_t (&a)[4], Tab tb, int t) {
 // 2^30 - 255, and with c30 = m ceil(2^30 / m) in [2^30, 2^30 + m) the value u = x + c30
 // lies in [0, 2^31) and keeps x's residue.  One v_mad_i32_i24 folds the high half:
 // y = u + (u >> 16)(c16 - 2^16) = (u >> 16) c16 + (u & 0xffff) <= (2^15 - 1) 254 + 65535
-// < 2^23, so 2y < 2^24 and q = mulhi24(2y, mu) = floor(y mu / 2^31) is floor(y/m) or
-// one less; r = y - q m lies in [0, 2m).
+// < 2^23, then y mod m by the exact quotient.
 OZ2_HD uint32_t residue_m30(uint32_t x, uint32_t m, uint32_t mu, uint32_t c16, uint32_t c30) {
     const uint32_t u = x + c30;
     const uint32_t y = mad_i24(static_cast<int32_t>(u >> 16), static_cast<int32_t>(c16) - 65536, u);
-    const uint32_t q = mulhi24(y + y, mu);
-    const uint32_t r = mad_i24(static_cast<int32_t>(q), -static_cast<int32_t>(m), y);
-    return umin(r, r - m);
+    return mod_m23(y, m, mu);
 }
 template <typename Tab>
 OZ2_HD uint32_t acc_residue(uint32_t x, Tab tb, int t) {

@@ -157,7 +157,7 @@ void make_tabsets(TabSet* out) {
             const uint32_t m = p.m[t];
             const uint32_t h = (m - 1) / 2;
             tb.m[t] = m;
-            tb.mu[t] = static_cast<uint32_t>((uint64_t{1} << 31) / m);
+            tb.mu[t] = static_cast<uint32_t>(((uint64_t{1} << 31) + m - 1) / m);
             tb.h[t] = h;
             tb.c16[t] = pow2_mod(16, m);
             tb.off[0][t] = (m - pow2_mod(62, m) + h) % m;

@@ -83,7 +83,7 @@ struct TabSet {
     int32_t L16;                            // 16-bit limbs of M: 2 * NL
     // split (hpp:202-220): Barrett constants and symmetric offsets
     uint32_t m[kMaxModuli];
-    uint32_t mu[kMaxModuli];                // floor(2^31 / m): Barrett constant for 24-bit operands
+    uint32_t mu[kMaxModuli];                // ceil(2^31 / m): exact quotients of values below 2^23 (mod_m23)
     uint32_t h[kMaxModuli];                 // (m - 1) / 2
     uint32_t c16[kMaxModuli];               // 2^16 mod m
     uint32_t off[2][kMaxModuli];            // (m - 2^OFF mod m + h) mod m, OFF = 62 (narrow), 104 (wide)

@@ -76,6 +76,15 @@ void emu_split_cplx(const double* re, const double* im, const int32_t* e, int64_
     else split_cplx_form<math::Limbs::Octets8>(re, im, e, n, s, conj, planes);
 }
 
+// exact-quotient reduction of modulus t of s over every v < 2^23: entries that differ from v mod m
+int64_t emu_mod_m23_mismatches(int s, int t) {
+    const TabSet* tb = tables() + s;
+    const uint32_t m = tb->m[t];
+    int64_t bad = 0;
+    for (uint32_t v = 0; v < (1u << 23); ++v) bad += math::mod_m23(v, m, tb->mu[t]) != v % m;
+    return bad;
+}
+
 // GEMM epilogue: residues of int32 sums, out[t * n + i] in [0, m_t)
 void emu_acc_residue(const int32_t* x, int64_t n, int s, uint8_t* out) {
     const TabSet* tb = tables() + s;

@@ -93,6 +93,21 @@ def test_split_complex_vs_oracle(s, tk, conj, rng):
             assert np.array_equal(out[g, t].astype(np.int64) & 0xFF, want & 0xFF), (t, g)
 
 
+def test_exact_quotient_every_value():
+    # mod_m23 (split and GEMM epilogue): q = mulhi24(2v, ceil(2^31 / m)) is floor(v / m)
+    # for every v < 2^23 and every modulus of every table (m = 256 never takes this path)
+    _emu.emu_mod_m23_mismatches.restype = ctypes.c_int64
+    _emu.emu_mod_m23_mismatches.argtypes = [i32, i32]
+    seen = set()
+    for s in range(2, 25):
+        for t, m in enumerate(O.moduli(s)):
+            if m == 256 or m in seen:
+                continue
+            seen.add(m)
+            assert _emu.emu_mod_m23_mismatches(s, t) == 0, m
+    assert len(seen) == 25
+
+
 @pytest.mark.parametrize("s", [2, 7, 14, 16, 24])
 def test_acc_residue(s, rng):
     # documented domain |x| < 2^30 - 255 (GEMM accumulators of one k-chunk: |x| <= 65408 * 2^14),
