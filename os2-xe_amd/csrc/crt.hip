@@ -64,8 +64,51 @@ __device__ __forceinline__ void crt_body(const CrtArgs& a, uint32_t bx) {
             }
         }
     }
+    if (CPLX && a.kch == 1) {
+        // the same for the three 3M groups, in two halves of the moduli to bound the
+        // words held at once; Re = P0 - P1 and Im = P2 - P0 - P1 reduced into [0, m)
+        const int64_t gs = static_cast<int64_t>(s) * a.plane_r;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            constexpr int H = kMaxModuli / 2;
+            const int t0 = half * H;
+            if (t0 >= s) break;
+            uint32_t w0[H], w1[H], w2[H];
+#pragma unroll
+            for (int u = 0; u < H; ++u) {
+                const uint8_t* b = base + (t0 + u) * a.plane_r;
+                const bool in = t0 + u < s;
+                w0[u] = in ? *reinterpret_cast<const uint32_t*>(b) : 0u;
+                w1[u] = in ? *reinterpret_cast<const uint32_t*>(b + gs) : 0u;
+                w2[u] = in ? *reinterpret_cast<const uint32_t*>(b + 2 * gs) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < H; u += 2) {
+                if (t0 + u < s) {
+                    const uint32_t m[2] = {tb->m[t0 + u], tb->m[t0 + u + 1]};  // 0 past s (residues 0 there)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        uint32_t xr[2], xi[2];
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const uint32_t x0 = (w0[u + h] >> (8 * e)) & 0xFFu;
+                            const uint32_t x1 = (w1[u + h] >> (8 * e)) & 0xFFu;
+                            const uint32_t x2 = (w2[u + h] >> (8 * e)) & 0xFFu;
+                            uint32_t r = x0 + m[h] - x1;  // [0, 2m)
+                            xr[h] = r >= m[h] ? r - m[h] : r;
+                            r = x2 + 2 * m[h] - x0 - x1;  // [0, 3m)
+                            r = r >= m[h] ? r - m[h] : r;
+                            xi[h] = r >= m[h] ? r - m[h] : r;
+                        }
+                        re[e].add2(tb, (t0 + u) >> 1, xr[0], xr[1]);
+                        im[e].add2(tb, (t0 + u) >> 1, xi[0], xi[1]);
+                    }
+                }
+            }
+        }
+    }
     // moduli in pairs (2 residues per dot2 step of the limb sums); r = 0 past s
-    for (int t = 0; t < s && (CPLX || a.kch > 1); t += 2) {
+    for (int t = 0; t < s && a.kch > 1; t += 2) {
         const bool two = t + 1 < s;
         const uint32_t m0 = tb->m[t], m1 = two ? tb->m[t + 1] : 1u;
         if (!CPLX) {

@@ -8,8 +8,8 @@
 //
 // Both operands are written in one blocked plane layout (plane_offset: 1 KB
 // blocks of 16 rows x 64 k), so the GEMM copies A and B tiles the same way.
-// A thread's 16 (real) or 4 (complex) k lie in one 16-byte chunk of a block
-// (block_k order), so each plane store is one aligned vector store.  Rows of op(X)
+// A thread's 16 k lie in one 16-byte chunk of a block (block_k order), so each
+// plane store is one aligned vector store.  Rows of op(X)
 // that are strided in memory (op(A) = N, op(B) = T) are read column-by-column
 // with lanes along rows, so no separate transpose pass is needed.
 #include <algorithm>
@@ -144,16 +144,31 @@ __device__ __forceinline__ uint64_t load_bits(const double* __restrict__ X, int6
     return static_cast<uint64_t>(__double_as_longlong(CPLX ? X[2 * i + part] : X[i]));
 }
 
-// Real split: each thread owns one 16-byte chunk of one row (16 k, block_k order) ->
-// one 16-byte store per plane.
-// KCONTIG: row r is contiguous (X[p + r*ldx]); block = 256 chunks along the row(s).
-// !KCONTIG: rows strided (X[r + p*ldx]); block = 32 rows x 8 chunks (128 k).
-// All loads are issued before any arithmetic and stay inside the matrix: padding
-// rows read row 0 and k past the end reads k - 1, and those values are then zeroed.
-template <bool WIDE, bool KCONTIG>
-__device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, uint32_t bx, uint32_t by) {
-    const auto tb = const_tab(a.tab);
-    int64_t r, ck;
+// Thread slot of a split: Q bytes (Q = 16: a whole 16-byte chunk, Q = 8: half of one) of
+// one row of the planes.  ck counts the slots along the row.
+template <int Q>
+struct SplitSlot {
+    static_assert(Q == 16 || Q == 8, "a slot is a chunk or half a chunk");
+    int64_t blk;  // 64-k block
+    int c;        // chunk in the block
+    int q0;       // first byte of the chunk filled by this slot
+    int64_t kb;   // byte q of the chunk holds k = kb + 8 (q >> 1) + (q & 1)
+    __device__ __forceinline__ explicit SplitSlot(int64_t ck) {
+        constexpr int per_blk = 64 / Q;
+        blk = ck / per_blk;
+        const int ib = static_cast<int>(ck % per_blk);
+        c = ib / (16 / Q);
+        q0 = (ib % (16 / Q)) * Q;
+        kb = blk * 64 + 2 * c;
+    }
+    __device__ __forceinline__ int64_t k_of(int q) const { return kb + 8 * ((q0 + q) >> 1) + ((q0 + q) & 1); }
+};
+
+// (row, slot) of a thread.  KCONTIG: row r is contiguous (X[p + r*ldx]); a block
+// covers 256 slots along the row(s).  !KCONTIG: rows strided (X[r + p*ldx]); a block
+// is 32 rows x 8 slots, lanes along rows.
+template <bool KCONTIG>
+__device__ __forceinline__ void split_thread(int lg_tpr, uint32_t bx, uint32_t by, int64_t& r, int64_t& ck) {
     if (KCONTIG) {
         const int tpr = 1 << lg_tpr;  // threads per row in a block
         r = static_cast<int64_t>(bx) * (256 >> lg_tpr) + (threadIdx.x >> lg_tpr);
@@ -162,34 +177,49 @@ __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, 
         r = static_cast<int64_t>(bx) * 32 + (threadIdx.x & 31);
         ck = static_cast<int64_t>(by) * 8 + (threadIdx.x >> 5);
     }
-    const int64_t nck = a.kp >> 4;
-    if (r >= a.rows_pad || ck >= nck) return;
-    const int64_t blk = ck >> 2;
-    const int c = static_cast<int>(ck & 3);
-    const int64_t kb = blk * 64 + 2 * c;  // byte q of the chunk holds k = kb + 8 (q >> 1) + (q & 1)
+}
+
+// Q bytes packed little-endian from 32-bit words: one Q-byte store
+template <int Q>
+__device__ __forceinline__ void store_bytes(int8_t* p, const uint32_t (&w)[4]) {
+    if (Q == 16) *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+    else *reinterpret_cast<uint2*>(p) = make_uint2(w[0], w[1]);
+}
+
+// Real split: Q bytes (Q of the k of one row, block_k order) per thread -> one
+// Q-byte store per plane.  All loads are issued before any arithmetic and stay
+// inside the matrix: padding rows read row 0 and k past the end reads k - 1, and
+// those values are then zeroed.
+template <bool WIDE, bool KCONTIG, int Q>
+__device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, uint32_t bx, uint32_t by) {
+    const auto tb = const_tab(a.tab);
+    int64_t r, ck;
+    split_thread<KCONTIG>(lg_tpr, bx, by, r, ck);
+    if (r >= a.rows_pad || ck >= a.kp / Q) return;
+    const SplitSlot<Q> sl(ck);
     const bool rowok = r < a.rows;
     const int32_t ex = rowok ? a.ex[r] : 0;
     const bool rowzero = !rowok || ex == INT32_MIN;
     const int e = rowzero ? 0 : ex;
     const int64_t rl = rowok ? r : 0;
 
-    uint64_t bits[16];
-    const bool full = blk * 64 + 64 <= a.k;
+    uint64_t bits[Q];
+    const bool full = sl.blk * 64 + 64 <= a.k;
     const double* rowp = a.X + rl * a.ldx;
-    if (KCONTIG && full && (reinterpret_cast<uintptr_t>(rowp + kb) & 15) == 0) {
+    if (KCONTIG && full && (reinterpret_cast<uintptr_t>(rowp + sl.kb) & 15) == 0) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const uint4 v = *reinterpret_cast<const uint4*>(rowp + kb + 8 * i);
+        for (int i = 0; i < Q / 2; ++i) {
+            const uint4 v = *reinterpret_cast<const uint4*>(rowp + sl.k_of(2 * i));
             bits[2 * i] = static_cast<uint64_t>(v.x) | (static_cast<uint64_t>(v.y) << 32);
             bits[2 * i + 1] = static_cast<uint64_t>(v.z) | (static_cast<uint64_t>(v.w) << 32);
         }
     } else if (full) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) bits[q] = load_bits<KCONTIG, false>(a.X, a.ldx, rl, kb + 8 * (q >> 1) + (q & 1), 0);
+        for (int q = 0; q < Q; ++q) bits[q] = load_bits<KCONTIG, false>(a.X, a.ldx, rl, sl.k_of(q), 0);
     } else {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int64_t p = kb + 8 * (q >> 1) + (q & 1);
+        for (int q = 0; q < Q; ++q) {
+            const int64_t p = sl.k_of(q);
             bits[q] = load_bits<KCONTIG, false>(a.X, a.ldx, rl, p < a.k ? p : a.k - 1, 0);
             bits[q] = p < a.k ? bits[q] : 0;
         }
@@ -197,87 +227,87 @@ __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, 
 
     // narrow values: the 8 bytes of the offset value, wide ones: its 14 bytes
     constexpr Limbs F = WIDE ? Limbs::Octets14 : Limbs::Octets8;
-    uint32_t limbs[16][4];
+    uint32_t limbs[Q][4];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) to_limbs<F>(rowzero ? 0 : bits[q], e, false, limbs[q]);
-    int8_t* out = a.P + chunk_offset(r, blk, c, a.kp);
+    for (int q = 0; q < Q; ++q) to_limbs<F>(rowzero ? 0 : bits[q], e, false, limbs[q]);
+    int8_t* out = a.P + chunk_offset(r, sl.blk, sl.c, a.kp) + sl.q0;
     for (int t = 0; t < a.s; ++t) {
-        uint32_t w[4];
+        uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            uint32_t acc = 0;
+        for (int g = 0; g < Q / 4; ++g)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc |= sym_byte<F>(limbs[4 * g + q], tb, t) << (8 * q);
-            w[g] = acc;
-        }
-        *reinterpret_cast<uint4*>(out + t * a.plane) = make_uint4(w[0], w[1], w[2], w[3]);
+            for (int q = 0; q < 4; ++q) w[g] |= sym_byte<F>(limbs[4 * g + q], tb, t) << (8 * q);
+        store_bytes<Q>(out + t * a.plane, w);
     }
 }
 
 // Complex split: three plane groups Re, Im (negated for conj), Re+Im, each
-// reduced to the symmetric range [-h, m-1-h].  4 bytes of one chunk per thread
-// (k = kb, kb + 1, kb + 8, kb + 9 in block_k order).
-template <bool WIDE, bool KCONTIG, bool CONJ>
+// reduced to the symmetric range [-h, m-1-h]; slots, k order and loads as in the
+// real split, one Q-byte store per plane of each group.
+template <bool WIDE, bool KCONTIG, bool CONJ, int Q>
 __device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, uint32_t bx, uint32_t by) {
     const auto tb = const_tab(a.tab);
     int64_t r, ck;
-    if (KCONTIG) {
-        const int tpr = 1 << lg_tpr;
-        r = static_cast<int64_t>(bx) * (256 >> lg_tpr) + (threadIdx.x >> lg_tpr);
-        ck = static_cast<int64_t>(by) * tpr + (threadIdx.x & (tpr - 1));
-    } else {
-        r = static_cast<int64_t>(bx) * 32 + (threadIdx.x & 31);
-        ck = static_cast<int64_t>(by) * 8 + (threadIdx.x >> 5);
-    }
-    const int64_t nck = a.kp >> 2;
-    if (r >= a.rows_pad || ck >= nck) return;
-    const int64_t blk = ck >> 4;
-    const int c = static_cast<int>((ck >> 2) & 3), w = static_cast<int>(ck & 3);
-    const int64_t kb = blk * 64 + 16 * w + 2 * c;  // bytes 4w + q: k = kb + 8 (q >> 1) + (q & 1)
+    split_thread<KCONTIG>(lg_tpr, bx, by, r, ck);
+    if (r >= a.rows_pad || ck >= a.kp / Q) return;
+    const SplitSlot<Q> sl(ck);
     const bool rowok = r < a.rows;
     const int32_t ex = rowok ? a.ex[r] : 0;
     const bool rowzero = !rowok || ex == INT32_MIN;
     const int e = rowzero ? 0 : ex;
     const int64_t rl = rowok ? r : 0;
 
-    uint64_t br[4], bi[4];
+    uint64_t br[Q], bi[Q];
+    const bool full = sl.blk * 64 + 64 <= a.k;
+    const double* rowp = a.X + 2 * rl * a.ldx;
+    if (KCONTIG && full && (reinterpret_cast<uintptr_t>(rowp + 2 * sl.kb) & 15) == 0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int64_t p = kb + 8 * (q >> 1) + (q & 1);
-        const int64_t pc = p < a.k ? p : a.k - 1;
-        br[q] = load_bits<KCONTIG, true>(a.X, a.ldx, rl, pc, 0);
-        bi[q] = load_bits<KCONTIG, true>(a.X, a.ldx, rl, pc, 1);
-        const bool z = rowzero || p >= a.k;
-        br[q] = z ? 0 : br[q];
-        bi[q] = z ? 0 : bi[q];
+        for (int q = 0; q < Q; ++q) {
+            const uint4 v = *reinterpret_cast<const uint4*>(rowp + 2 * sl.k_of(q));
+            br[q] = static_cast<uint64_t>(v.x) | (static_cast<uint64_t>(v.y) << 32);
+            bi[q] = static_cast<uint64_t>(v.z) | (static_cast<uint64_t>(v.w) << 32);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int64_t p = sl.k_of(q);
+            const int64_t pc = p < a.k ? p : a.k - 1;
+            br[q] = load_bits<KCONTIG, true>(a.X, a.ldx, rl, pc, 0);
+            bi[q] = load_bits<KCONTIG, true>(a.X, a.ldx, rl, pc, 1);
+            br[q] = p < a.k ? br[q] : 0;
+            bi[q] = p < a.k ? bi[q] : 0;
+        }
     }
     constexpr Limbs F = WIDE ? Limbs::Octets14 : Limbs::Octets8;
-    uint32_t lr[4][4], li[4][4];
+    uint32_t lr[Q][4], li[Q][4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        to_limbs<F>(br[q], e, false, lr[q]);
-        to_limbs<F>(bi[q], e, false, li[q]);
+    for (int q = 0; q < Q; ++q) {
+        to_limbs<F>(rowzero ? 0 : br[q], e, false, lr[q]);
+        to_limbs<F>(rowzero ? 0 : bi[q], e, false, li[q]);
     }
-    int8_t* out = a.P + chunk_offset(r, blk, c, a.kp) + 4 * w;
+    int8_t* out = a.P + chunk_offset(r, sl.blk, sl.c, a.kp) + sl.q0;
     for (int t = 0; t < a.s; ++t) {
         const int32_t m = static_cast<int32_t>(tb->m[t]);
         const int32_t h = static_cast<int32_t>(tb->h[t]);
-        uint32_t w0 = 0, w1 = 0, w2 = 0;
+        uint32_t w0[4] = {0, 0, 0, 0}, w1[4] = {0, 0, 0, 0}, w2[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int32_t s0 = sym_value<F>(lr[q], tb, t);
-            int32_t s1 = sym_value<F>(li[q], tb, t);
-            if (CONJ) s1 = -s1;
-            int32_t ss = s0 + s1;
-            ss = ss > m - 1 - h ? ss - m : ss;
-            ss = ss < -h ? ss + m : ss;
-            w0 |= (static_cast<uint32_t>(s0) & 0xFFu) << (8 * q);
-            w1 |= (static_cast<uint32_t>(s1) & 0xFFu) << (8 * q);
-            w2 |= (static_cast<uint32_t>(ss) & 0xFFu) << (8 * q);
+        for (int g = 0; g < Q / 4; ++g) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int32_t s0 = sym_value<F>(lr[4 * g + q], tb, t);
+                int32_t s1 = sym_value<F>(li[4 * g + q], tb, t);
+                if (CONJ) s1 = -s1;
+                int32_t ss = s0 + s1;
+                ss = ss > m - 1 - h ? ss - m : ss;
+                ss = ss < -h ? ss + m : ss;
+                w0[g] |= (static_cast<uint32_t>(s0) & 0xFFu) << (8 * q);
+                w1[g] |= (static_cast<uint32_t>(s1) & 0xFFu) << (8 * q);
+                w2[g] |= (static_cast<uint32_t>(ss) & 0xFFu) << (8 * q);
+            }
         }
-        *reinterpret_cast<uint32_t*>(out + t * a.plane) = w0;
-        *reinterpret_cast<uint32_t*>(out + a.group + t * a.plane) = w1;
-        *reinterpret_cast<uint32_t*>(out + 2 * a.group + t * a.plane) = w2;
+        store_bytes<Q>(out + t * a.plane, w0);
+        store_bytes<Q>(out + a.group + t * a.plane, w1);
+        store_bytes<Q>(out + 2 * a.group + t * a.plane, w2);
     }
 }
 
@@ -287,11 +317,14 @@ struct SplitItem {
     int lg;
     uint32_t gx;
 };
+// bytes per thread slot: half chunks for wide complex values (their 2 x 16 octet words
+// per element would not fit 16 elements in registers), whole chunks otherwise
+constexpr int split_q(bool wide, bool cplx) { return wide && cplx ? 8 : 16; }
 template <int V>
 __device__ __forceinline__ void split_body(const SplitItem& it, uint32_t bx, uint32_t by) {
     constexpr bool W = V & 1, KC = (V >> 1) & 1, CX = (V >> 2) & 1, CJ = (V >> 3) & 1;
-    if constexpr (CX) split_cplx_body<W, KC, CJ>(it.a, it.lg, bx, by);
-    else split_real_body<W, KC>(it.a, it.lg, bx, by);
+    if constexpr (CX) split_cplx_body<W, KC, CJ, split_q(W, true)>(it.a, it.lg, bx, by);
+    else split_real_body<W, KC, split_q(W, false)>(it.a, it.lg, bx, by);
 }
 template <int V>
 __global__ void __launch_bounds__(256) split_kernel(SplitItem it) { split_body<V>(it, blockIdx.x, blockIdx.y); }
@@ -300,9 +333,9 @@ int split_variant(const SplitArgs& a) {
     return (a.wide ? 1 : 0) | (a.kcontig ? 2 : 0) | (a.cplx ? 4 : 0) | (a.cplx && a.conj ? 8 : 0);
 }
 
-// grid of one split: KCONTIG: rows x chunk groups (tpr threads per row); strided: 32 rows x 8 chunks
+// grid of one split: KCONTIG: rows x slot groups (tpr threads per row); strided: 32 rows x 8 slots
 SplitItem split_item(const SplitArgs& a, uint32_t& gy) {
-    const int64_t nck = a.kp / (a.cplx ? 4 : 16);
+    const int64_t nck = a.kp / split_q(a.wide, a.cplx);
     SplitItem it{a, 0, 0};
     if (a.kcontig) {
         int tpr = 1;
