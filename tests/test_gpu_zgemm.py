@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oz2
-from gpu_cases import bits, complex_case
+from gpu_cases import bits, complex_case, log, make_cmat, to_dev
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -73,3 +73,38 @@ def test_alpha_zero_scales_c(oz):
     want = C0.view(np.complex128) * (0.5 - 2j)
     assert np.allclose(got, want, rtol=0, atol=1e-15)
     O.precision_bits(8, 8)
+
+
+def _sampled_complex(oz, ta, tb, m, n, k, s, dist, ld_extra=0, seed=13, count=256):
+    """large ZGEMM (alpha = 1, beta = 0) checked at sampled entries against the closed form"""
+    import torch
+    rng = np.random.default_rng(seed)
+    ar, ac = (m, k) if ta == "N" else (k, m)
+    br, bc = (k, n) if tb == "N" else (n, k)
+    lda, ldb = ar + ld_extra, br + ld_extra
+    A = make_cmat(ar, ac, lda, dist, rng, 1.0).reshape(-1, order="F").view(np.float64).copy()
+    B = make_cmat(br, bc, ldb, dist, rng, 1.0).reshape(-1, order="F").view(np.float64).copy()
+    log(f"case large zgemm {ta}{tb} m={m} n={n} k={k} s={s} {dist} ld+{ld_extra}")
+    dC = to_dev(np.zeros(2 * m * n))
+    oz.zgemm(ta, tb, m, n, k, 1.0, to_dev(A), lda, to_dev(B), ldb, 0.0, dC, m, s)
+    torch.cuda.synchronize()
+    C = dC.cpu().numpy()
+    rows = rng.integers(0, m, count)
+    cols = rng.integers(0, n, count)
+    rows[:4] = [0, m - 1, 0, m - 1]
+    cols[:4] = [0, 0, n - 1, n - 1]
+    want = O.gemm_entries(ta, tb, m, n, k, A, lda, B, ldb, s, rows, cols, cplx=True).reshape(-1, 2)
+    got = np.stack([C[2 * (rows + cols * m)], C[2 * (rows + cols * m) + 1]], axis=1)
+    assert np.array_equal(bits(got), bits(want))
+
+
+def test_large_short_k_rectangular_tile(oz):
+    # k loops up to 1024: 256 x 128 tiles; ragged m, n; padded leading dimensions (odd ld:
+    # the contiguous-row split takes its element-wise path)
+    _sampled_complex(oz, "C", "N", 1100, 900, 700, 16, "phi", ld_extra=7)
+    _sampled_complex(oz, "N", "T", 1000, 1300, 1000, 14, "uniform")
+
+
+def test_large_wide_split(oz):
+    # s = 24: tk > 62, the wide complex split (half-chunk slots), long k: 256 x 256 tiles
+    _sampled_complex(oz, "T", "C", 600, 700, 3000, 24, "phi", ld_extra=1)
