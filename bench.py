@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per INT8 GEMM launch measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE passes,
 # each scaled by its reading for a 1 GiB copy kernel): profiles/dgemm8192_s16_hbm_traffic.txt
-MEASURED_GEMM_TRAFFIC = {(8192, 8192, 8192, 16): 14.09e9}
+MEASURED_GEMM_TRAFFIC = {(8192, 8192, 8192, 16): 14.80e9}
 
 
 def parse():
@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (the runtime's copy kernel): the "
+                         "known-size reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     ap.add_argument("--vendor-int8", action="store_true",
                     help="also time the vendor INT8 GEMM (hipBLASLt through torch._int_mm) on s GEMMs of the same "
                          "shape, int32 output, as the rate this GPU reaches with a library kernel")
@@ -349,6 +352,17 @@ def main():
     import torch
     import oz2
 
+    if args.calibrate_copy:
+        nbytes = 1 << 30
+        src = torch.randint(0, 256, (nbytes,), dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device-to-device copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                          "launches": 3}))
+        return
     if args.workload == "zgemm4096":
         print(json.dumps(zgemm_line(args, torch)))
         return
