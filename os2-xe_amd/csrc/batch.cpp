@@ -2,6 +2,7 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
@@ -286,6 +287,27 @@ void BatchPlan::run_gemm_only(hipStream_t st) {
     hip_check(hipGetLastError(), "batch gemm launch");
 }
 
+namespace {
+// INT8 GEMM rate (TOPS) of a macro tile at k-loop length kp, measured on MI355X with s = 16
+// random planes of 4096 x 4096, k = 256 .. 8192: piecewise linear in log2(kp)
+double tile_rate(k::Tile t, int64_t kp) {
+    static const double lk[6] = {8, 9, 10, 11, 12, 13};  // log2 k
+    static const double rate[4][6] = {
+        {780, 1198, 1456, 1751, 2121, 2396},  // 256 x 256
+        {688, 1202, 1646, 1866, 1976, 1933},  // 128 x 128
+        {558, 817, 1082, 1269, 1351, 1179},   // 64 x 64
+        {955, 1376, 1770, 2005, 2051, 2089},  // 256 x 128
+    };
+    const double* r = rate[static_cast<int>(t)];
+    const double x = std::log2(static_cast<double>(std::max<int64_t>(kp, 128)));
+    if (x <= lk[0]) return r[0] * std::pow(2.0, x - lk[0]);  // shorter k: epilogue-bound, rate ~ k
+    if (x >= lk[5]) return r[5];
+    int i = 0;
+    while (x > lk[i + 1]) ++i;
+    return r[i] + (r[i + 1] - r[i]) * (x - lk[i]);
+}
+}  // namespace
+
 TuneResult autotune_batch(Engine& eng, const std::vector<GemmDesc>& d, const std::vector<int>& s, int reps) {
     TuneResult best;
     best.ms = 1e300;
@@ -325,6 +347,25 @@ TuneResult autotune_batch(Engine& eng, const std::vector<GemmDesc>& d, const std
         std::vector<int> tiles;
     };
     std::vector<Cand> cands{{"heuristic", {}}};
+    // per problem, the tile with the least padded work / measured rate at its k: in a grouped
+    // launch the union of the problems fills the GPU, so each problem only needs its fastest tile
+    {
+        std::vector<int> t(d.size());
+        for (size_t i = 0; i < d.size(); ++i) {
+            const int64_t kp = k_chunk_len(round_up(std::max<int64_t>(d[i].k, 1), kKStep));
+            double best_t = 0;
+            for (k::Tile c : {k::Tile::T256, k::Tile::T256x128, k::Tile::T128, k::Tile::T64}) {
+                const double work = static_cast<double>(round_up(d[i].m, k::tile_m(c))) *
+                                    static_cast<double>(round_up(d[i].n, k::tile_n(c))) * static_cast<double>(kp);
+                const double est = work / tile_rate(c, kp);
+                if (best_t == 0 || est < best_t) {
+                    best_t = est;
+                    t[i] = k::tile_code(c);
+                }
+            }
+        }
+        cands.push_back({"per-problem tile by padded work / rate", t});
+    }
     for (int64_t kr : {int64_t{0}, int64_t{1024}})
         for (auto [a, b] : {std::pair<int64_t, int64_t>{192, 64}, {256, 96}, {384, 128}, {512, 160}, {768, 256},
                             {inf, 128}, {inf, 64}})

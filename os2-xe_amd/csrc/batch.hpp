@@ -80,7 +80,8 @@ private:
 // plans (the union of all problems fills the GPU, so the per-problem heuristic is not
 // the best choice): tile policies tile(min(m, n)) = 256 if >= a, 128 if >= b, else 64,
 // each also with 256 x 128 tiles for the problems whose k loop is at most 1024, plus
-// the single-problem heuristic, each with and without the persistent schedule; then
+// the single-problem heuristic and a per-problem choice by padded work / measured tile
+// rate at the problem's k, each with and without the persistent schedule; then
 // problem order instead of largest first for the winner.
 struct TuneResult {
     std::vector<int> tiles;
