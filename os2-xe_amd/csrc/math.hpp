@@ -281,25 +281,31 @@ struct CrtAcc {
                 cy = static_cast<uint32_t>(s >> 32);
             }
         }
-        // leading nonzero word h; 64-bit window from words h, h-1, h-2 plus a sticky bit
-        int h = -1;
-        for (int l = 0; l < NL; ++l) h = w[l] ? l : h;
-        if (h < 0) return 0.0;
-        uint32_t w0 = 0, w1 = 0, w2 = 0;
-        bool below = false;
-        for (int l = 0; l < NL; ++l) {
-            w0 = (l == h) ? w[l] : w0;
-            w1 = (l == h - 1) ? w[l] : w1;
-            w2 = (l == h - 2) ? w[l] : w2;
-            below = below || (l < h - 2 && w[l] != 0);
+        // |C'| as G 64-bit groups; the window is the top 64 bits of the leading nonzero group
+        // and the one below it, every lower bit folded into a sticky bit
+        constexpr int G = (NL + 1) / 2;
+        static_assert(G <= 3, "at most 6 words");
+        uint64_t g[G];
+        for (int i = 0; i < G; ++i)
+            g[i] = w[2 * i] | (2 * i + 1 < NL ? static_cast<uint64_t>(w[2 * i + 1]) << 32 : 0);
+        uint64_t hi = g[G - 1], lo = G >= 2 ? g[G - 2] : 0;
+        bool below = G == 3 && g[0] != 0;
+        int base = 64 * (G - 1);  // log2 weight of hi's lowest bit
+        for (int step = 0; step + 1 < G; ++step) {  // step down past zero groups
+            const bool z = hi == 0;
+            const int idx = G - 2 - step;            // lo is group idx
+            hi = z ? lo : hi;
+            lo = z ? (idx >= 1 ? g[idx - 1] : 0) : lo;
+            base -= z ? 64 : 0;
+            below = z ? false : below;               // with G <= 3 nothing lies below the new lo
         }
-        const int z = clz32(w0);
-        const uint64_t hi64 = (static_cast<uint64_t>(w0) << 32) | w1;
-        uint64_t top = z ? (hi64 << z) | (w2 >> (32 - z)) : hi64;
-        const bool sticky = below || ((z ? (w2 << z) : w2) != 0);
-        top |= sticky ? 1u : 0u;
-        // the window's lsb weighs 2^(32(h-1) - z); one RNE here, a second only if subnormal
-        const double v = ldexp_(static_cast<double>(top), 32 * (h - 1) - z - shift);
+        if (hi == 0) return 0.0;
+        const int z = clz64(hi);
+        uint64_t top = z ? (hi << z) | (lo >> (64 - z)) : hi;
+        const uint64_t rest = z ? (lo << z) : lo;
+        top |= (below || rest != 0) ? 1u : 0u;
+        // the window's lsb weighs 2^(base - z); one RNE here, a second only if subnormal
+        const double v = ldexp_(static_cast<double>(top), base - z - shift);
         return neg ? -v : v;
     }
 };
