@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-native", action="store_true",
+                    help="skip the native FP64 GEMM (rocBLAS) timing and accuracy on the same inputs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (the runtime's copy kernel): the "
                          "known-size reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -247,6 +249,9 @@ def zgemm_line(args, torch):
     out["accuracy"] = {"ulp_max": float(e.max()), "ulp_mean": float(e.mean()),
                        "cr_frac": float((Cv == hi + lo).double().mean()),
                        "reference": "double-double ZGEMM on the GPU, per real component, alpha=1 beta=0"}
+    if not args.no_native:
+        out["native_fp64"] = native_fp64(torch, m, n, k, dA, dB, cplx=True, op_a="C", lda=lda)
+        out["x_native_fp64"] = round(out["value"] / out["native_fp64"]["tflops"], 3)
     oz.close()
     return out
 
@@ -312,6 +317,50 @@ def cpu_baseline(m, n, k, s):
             "seconds": dt,
             "sample": f"rows 0..{rows - 1} of the {m}x{n}x{k} s={s} DGEMM (alpha=1.5, beta=0.5): {rows}x{n}x{k}, "
                       f"oracle residue route (split, exact int8 GEMM, CRT) with OpenMP"}
+
+
+def native_fp64(torch, m, n, k, dA, dB, cplx=False, op_a="N", lda=None, reps=5):
+    """The GPU's native FP64 GEMM (rocBLAS through torch.matmul) on the same column-major inputs:
+    best time of reps, TFLOP/s (2mnk, complex 8mnk) and max-ulp against double-double (alpha=1, beta=0)."""
+    from oz2 import check
+    if not cplx:
+        A = dA.view(k, m).t()  # column-major m x k
+        B = dB.view(n, k).t()
+    else:
+        ld = lda or (k if op_a != "N" else m)
+        Ac = torch.view_as_complex(dA.view(-1, 2))
+        A = Ac.view(m, ld)[:, :k].conj() if op_a == "C" else Ac.view(k, ld)[:, :m].t()  # op(A), m x k
+        B = torch.view_as_complex(dB.view(-1, 2)).view(n, k).t()
+    C = A @ B
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        C = A @ B
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    flops = (8.0 if cplx else 2.0) * m * n * k
+    out = {"library": "rocBLAS through torch.matmul (" + ("complex128" if cplx else "float64") + ")",
+           "ms": round(best, 4), "tflops": round(flops / best / 1e9, 2)}
+    hi = torch.empty((2 if cplx else 1) * m * n, dtype=torch.float64, device="cuda")
+    lo = torch.empty_like(hi)
+    if not cplx:
+        rc = check.lib().oz2chk_dd_gemm(None, 0, 0, m, n, k, dA.data_ptr(), m, dB.data_ptr(), k, hi.data_ptr(),
+                                        lo.data_ptr(), 0)
+        Cv = C.t().contiguous().view(-1)
+    else:
+        rc = check.lib().oz2chk_dd_gemm(None, 2 if op_a == "C" else 0, 0, m, n, k, dA.data_ptr(), lda or m,
+                                        dB.data_ptr(), k, hi.data_ptr(), lo.data_ptr(), 1)
+        Cv = torch.view_as_real(C.t().contiguous()).reshape(-1)
+    assert rc == 0
+    torch.cuda.synchronize()
+    e = check.ulp_errors(Cv, hi, lo, hi.numel(), hi.numel())
+    out["ulp_max"] = float(e.max())
+    out["cr_frac"] = float((Cv == hi + lo).double().mean())
+    return out
 
 
 def vendor_int8(torch, m, n, k, s, reps=5):
@@ -441,6 +490,9 @@ def main():
     }
     if not args.no_accuracy:
         out["accuracy"] = accuracy(oz, torch, dA, dB, m, n, k, s)
+    if not args.no_native:
+        out["native_fp64"] = native_fp64(torch, m, n, k, dA, dB)
+        out["x_native_fp64"] = round(value / out["native_fp64"]["tflops"], 3)
     if not args.no_sweep:
         sweep = {}
         for sv in (8, 12, 16):

@@ -21,6 +21,7 @@ def lib():
         L = ctypes.CDLL(_PATH)
         vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
         L.oz2chk_dd_gemm.argtypes = [vp, i32, i32, i64, i64, i64, vp, i64, vp, i64, vp, vp, i32]
+        L.oz2chk_gen.argtypes = [ctypes.c_char_p, i64, i64, ctypes.c_uint64, i32, vp]
         _lib = L
     return _lib
 
@@ -55,3 +56,28 @@ def metrics(C, hi, lo, m, ldc):
     n = hi.numel() // max(1, m)
     Cv = C.reshape(-1)[: ldc * n].reshape(n, ldc)[:, :m].reshape(-1)
     return {"ulp_max": float(e.max()), "ulp_mean": float(e.mean()), "cr_frac": float((Cv == hi + lo).double().mean())}
+
+
+def gen(dist, rows, cols, seed, is_a):
+    """rows x cols column-major float64 (numpy) input of the accuracy benchmark: the reference's
+    generator (bench_accuracy.cpp:41-69) for dist in urand, phiX, sincos, spike, cancel"""
+    import numpy as np
+    X = np.empty(rows * cols, dtype=np.float64)
+    rc = lib().oz2chk_gen(dist.encode(), rows, cols, seed, 1 if is_a else 0, X.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oz2: unknown distribution {dist!r}")
+    return X
+
+
+def full_metrics(C, hi, lo, absab):
+    """the reference's accuracy metrics (bench_accuracy.cpp:71-94) of C (flat, same layout as
+    hi, lo, absab): ulp_max, ulp_mean, cr_frac, ab_max = max err / sum |a||b|, fro_rel =
+    ||err||_F / ||hi||_F"""
+    err = ((C - hi) - lo).abs()
+    a = hi.abs()
+    ulp = torch.where(hi == 0, torch.full_like(a, 5e-324), torch.nextafter(a, torch.full_like(a, float("inf"))) - a)
+    eu = err / ulp
+    pos = absab > 0
+    ab = (err[pos] / absab[pos]).max() if bool(pos.any()) else torch.zeros((), dtype=torch.float64)
+    return {"ulp_max": float(eu.max()), "ulp_mean": float(eu.mean()), "cr_frac": float((C == hi + lo).double().mean()),
+            "ab_max": float(ab), "fro_rel": float(torch.sqrt((err * err).sum() / (hi * hi).sum()))}

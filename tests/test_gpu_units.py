@@ -161,3 +161,21 @@ def test_crt_epilogue_and_nan():
 def _fma(a, b, c):
     from fractions import Fraction
     return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def test_accuracy_harness_rows():
+    # the accuracy benchmark (reference benchmarks/bench_accuracy.cpp): CSV rows per method
+    import io
+    from oz2 import bench_accuracy
+    for cplx in (False, True):
+        buf = io.StringIO()
+        argv = ["--k", "300", "--s", "6,16", "--dist", "urand", "--mn", "40"] + (["--complex"] if cplx else [])
+        bench_accuracy.main(argv, out=buf)
+        lines = buf.getvalue().strip().split("\n")
+        assert lines[0] == "dist,k,method,s,tk,ulp_max,ulp_mean,cr_frac,ab_max,fro_rel"
+        rows = [line.split(",") for line in lines[1:]]
+        methods = [r[2] for r in rows]
+        assert methods == (["zgemm_gpu", "ozaki2_3m", "ozaki2_3m"] if cplx else ["dgemm_gpu", "ozaki2", "ozaki2"])
+        s16 = rows[2]
+        assert s16[3] == "16" and float(s16[7]) == 1.0 and float(s16[5]) <= 0.5  # correctly rounded at s = 16
+        assert float(rows[1][5]) > 1.0  # s = 6 keeps too few bits
