@@ -10,8 +10,9 @@ and the same CSV columns:
 
 with method dgemm_gpu (the GPU's native FP64 GEMM: rocBLAS through torch.matmul),
 dgemm_cpu (numpy's BLAS, with --cpu), ozaki2 (this library, one row per s whose plan
-admits k); complex: zgemm_gpu, zgemm_cpu, ozaki2_3m.  The reference's ozaki1 rows
-(Ozaki Scheme I) have no counterpart here.  m = n = mn, alpha = 1, beta = 0, op = N.
+admits k), ozaki1 (the Ozaki Scheme I baseline of oz2/ozaki1.py, one row per --L, with
+s = L(L+1)/2 GEMMs and tk = 7L - 2); complex: zgemm_gpu, zgemm_cpu, ozaki2_3m.
+m = n = mn, alpha = 1, beta = 0, op = N.
 
     python -m oz2.bench_accuracy --k 256,4096 --s 8-24 --dist urand,phi2
 """
@@ -22,7 +23,7 @@ import numpy as np
 import torch
 
 import oz2
-from oz2 import check
+from oz2 import check, ozaki1
 
 
 def parse_ints(text):
@@ -44,7 +45,7 @@ def _row(out, dist, k, method, s, tk, met):
           f"{met['ab_max']:.4g},{met['fro_rel']:.4g}", file=out, flush=True)
 
 
-def run_real(oz, dist, k, m, n, svals, cpu, out):
+def run_real(oz, dist, k, m, n, svals, cpu, out, Ls=()):
     A = check.gen(dist, m, k, 11 + k, True)
     B = check.gen(dist, k, n, 77 + k, False)
     dA = torch.from_numpy(A).cuda()
@@ -69,6 +70,12 @@ def run_real(oz, dist, k, m, n, svals, cpu, out):
             continue
         oz.gemm("N", "N", m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, s)
         _row(out, dist, k, "ozaki2", s, tk, metrics(dC))
+    for L in Ls:
+        try:
+            ozaki1.gemm(m, n, k, dA, m, dB, k, dC, m, L)
+        except ValueError:
+            continue
+        _row(out, dist, k, "ozaki1", ozaki1.num_gemms(L), ozaki1.precision_bits(L), metrics(dC))
 
 
 def run_complex(oz, dist, k, m, n, svals, cpu, out):
@@ -119,14 +126,19 @@ def main(argv=None, out=sys.stdout):
     ap.add_argument("--mn", type=int, default=512)
     ap.add_argument("--cpu", action="store_true")
     ap.add_argument("--complex", action="store_true")
+    ap.add_argument("--L", default="4-9", help="Ozaki Scheme I slice counts (real runs; empty: none)")
     args = ap.parse_args(argv)
     ks, svals, dists = parse_ints(args.k), parse_ints(args.s), [d for d in args.dist.split(",") if d]
+    Ls = parse_ints(args.L)
     oz = oz2.Ozaki2()
     print("dist,k,method,s,tk,ulp_max,ulp_mean,cr_frac,ab_max,fro_rel", file=out, flush=True)
     try:
         for dist in dists:
             for k in ks:
-                (run_complex if args.complex else run_real)(oz, dist, k, args.mn, args.mn, svals, args.cpu, out)
+                if args.complex:
+                    run_complex(oz, dist, k, args.mn, args.mn, svals, args.cpu, out)
+                else:
+                    run_real(oz, dist, k, args.mn, args.mn, svals, args.cpu, out, Ls)
     finally:
         oz.close()
 

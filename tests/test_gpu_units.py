@@ -169,7 +169,7 @@ def test_accuracy_harness_rows():
     from oz2 import bench_accuracy
     for cplx in (False, True):
         buf = io.StringIO()
-        argv = ["--k", "300", "--s", "6,16", "--dist", "urand", "--mn", "40"] + (["--complex"] if cplx else [])
+        argv = ["--k", "300", "--s", "6,16", "--dist", "urand", "--mn", "40", "--L", ""] + (["--complex"] if cplx else [])
         bench_accuracy.main(argv, out=buf)
         lines = buf.getvalue().strip().split("\n")
         assert lines[0] == "dist,k,method,s,tk,ulp_max,ulp_mean,cr_frac,ab_max,fro_rel"
