@@ -23,6 +23,17 @@ void hip_check(hipError_t e, const char* what) {
 constexpr double kInt8Rate = 4.0e15;
 constexpr double kHbmRate = 5.0e12;
 constexpr double kLaunch = 6e-6;
+
+// Plane strides that are multiples of 64 KiB put the same offset of every plane
+// on the same HBM channels; the GEMM (one plane per XCD range) and the CRT (all
+// planes of one entry) read those offsets at the same time.  Tuning::pad_lda
+// staggers such strides by 4 KiB, the counterpart of the reference's ld pad
+// for cache-set conflicts (ozaki2_gpu.hpp:669-672).
+constexpr int64_t kPlanePadPeriod = 64 << 10;
+constexpr int64_t kPlanePad = 4 << 10;
+inline int64_t plane_stride(int64_t bytes, bool pad) {
+    return pad && bytes % kPlanePadPeriod == 0 ? bytes + kPlanePad : bytes;
+}
 }  // namespace
 
 struct Engine::Layout {
@@ -178,9 +189,9 @@ Engine::Layout Engine::layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int
     L.TN = k::tile_n(L.tile);
     L.mpad = round_up(mb, L.TM);
     L.npad = round_up(nb, L.TN);
-    L.planeA = L.mpad * L.kp;
-    L.planeB = L.npad * L.kp;
-    L.planeR = L.npad * L.mpad;
+    L.planeA = plane_stride(L.mpad * L.kp, tune_.pad_lda);
+    L.planeB = plane_stride(L.npad * L.kp, tune_.pad_lda);
+    L.planeR = plane_stride(L.npad * L.mpad, tune_.pad_lda);
     size_t off = 0;
     auto carve = [&](size_t bytes) {
         const size_t o = off;
@@ -362,7 +373,20 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
                 ga.k0 = c * klen;
                 ga.klen = std::min(klen, kp - ga.k0);
                 ga.R = R + c * L.planes * L.planeR;
-                k::igemm_residue(stream_, tile_, ga);
+                if (tune_.batched_gemm) {
+                    k::igemm_residue(stream_, tile_, ga);
+                    continue;
+                }
+                // one launch per plane (the reference's per-plane GEMM loop)
+                k::GemmArgs g1 = ga;
+                g1.nplanes = 1;
+                for (int t = 0; t < L.planes; ++t) {
+                    g1.A = ga.A + t * L.planeA;
+                    g1.B = ga.B + t * L.planeB;
+                    g1.R = ga.R + t * L.planeR;
+                    g1.plane0 = t;
+                    k::igemm_residue(stream_, tile_, g1);
+                }
             }
             record(Stage::Gemm);
             k::CrtArgs ca{};

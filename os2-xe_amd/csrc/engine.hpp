@@ -30,8 +30,8 @@ struct TileInfo {
 };
 
 struct Tuning {
-    bool batched_gemm = true;  // one launch for all planes (the only form here)
-    bool pad_lda = true;       // accepted for API parity; plane rows are padded to the GEMM tile anyway
+    bool batched_gemm = true;  // one GEMM launch for all planes; false: one launch per plane
+    bool pad_lda = true;       // stagger plane strides that are multiples of 64 KiB by 4 KiB (engine.cpp)
     bool pow2_tiles = true;    // include power-of-two tile candidates
     int gemm_tile = -1;        // -1 auto, else a tile code: 256 / 128 / 64 (square) or 256128 (256 x 128)
 };

@@ -64,6 +64,17 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
     return (b & 7) * (n >> 3) + (b >> 3);
 }
 
+// The same per plane: work items stay plane-major and each plane's tiles are
+// split over the XCDs in contiguous ranges, so all XCDs work on one plane at a
+// time and share its A and B planes in the last-level cache (with whole ranges
+// of planes per XCD, 8 plane pairs compete for it).
+__device__ __forceinline__ uint32_t plane_xcd_remap(uint32_t b, uint32_t tiles, uint32_t n) {
+    if ((tiles & 7) != 0) return xcd_remap(b, n);
+    const uint32_t plane = b / tiles;
+    const uint32_t r = b - plane * tiles;
+    return plane * tiles + (r & 7) * (tiles >> 3) + (r >> 3);
+}
+
 // Residues of one wave tile (WTM x WTN) staged through LDS: 4 residues per
 // 32-bit LDS store into a [col][row] byte tile whose 16-byte chunks are
 // XOR-permuted by column, then 16-byte global stores that write whole runs of
@@ -72,7 +83,7 @@ template <int FM, int FN>
 __device__ __forceinline__ void residue_epilogue(const GemmArgs& p, v16i (&acc)[FM][FN], uint32_t plane,
                                                  int64_t row0, int64_t col0, char* wlds, int lane) {
     constexpr int WTM = FM * 32, WTN = FN * 32, NCH = WTM / 16;
-    const int t = static_cast<int>(plane % static_cast<uint32_t>(p.s));
+    const int t = static_cast<int>((static_cast<uint32_t>(p.plane0) + plane) % static_cast<uint32_t>(p.s));
     const auto tb = const_tab(p.tab);
     const uint32_t m = tb->m[t];
     const int fr = lane & 31, fc = lane >> 5;
@@ -259,7 +270,8 @@ template <int BM, int BN, int MINB, int NS>
 __global__ void __launch_bounds__(256, MINB) igemm_residue_kernel(GemmArgs p) {
     __shared__ v4i smem[TileShape<BM, BN, NS>::SMEM / 16];
     uint32_t plane, tm, tn;
-    tile_coords(p, xcd_remap(blockIdx.x, gridDim.x), plane, tm, tn);
+    const uint32_t tiles = static_cast<uint32_t>(p.tiles_m) * static_cast<uint32_t>(p.tiles_n);
+    tile_coords(p, plane_xcd_remap(blockIdx.x, tiles, gridDim.x), plane, tm, tn);
     gemm_tile<BM, BN, NS>(p, plane, tm, tn, smem);
 }
 

@@ -73,7 +73,8 @@ struct GemmArgs {
     int64_t ldr;          // column stride of residues (= mpad)
     int64_t k0, klen;     // k range (multiples of 128)
     int tiles_m, tiles_n;
-    int nplanes;          // s or 3s
+    int nplanes;          // planes of this launch (s or 3s; 1 for per-plane launches)
+    int plane0;           // global index of this launch's first plane: its modulus is (plane0 + plane) mod s
     int s;
     const TabSet* tab;    // device tables of this s
 };
