@@ -179,33 +179,54 @@ OZ2_HD uint32_t mod_m23(uint32_t v, uint32_t m, uint32_t mu) {
     return mad_i24(static_cast<int32_t>(q), -static_cast<int32_t>(m), v);
 }
 
-// (x + h) mod m in [0, m) for modulus index t (m <= 255) from the octets of u = 2^OFF + x:
-// v = (h - 2^OFF mod m) + sum_j byte_j(u) (2^(8 j) mod m), one v_dot4 per word;
-// v <= 14 * 255 * 254 + 254 < 2^20.
+// (v mod m) - h for w = v - h (mod 2^32), v < 2^23: the exact quotient of v = w + h
+// takes (w << 1) + 2h (one v_lshl_add_u32) and w - q m = (v mod m) - h needs no
+// separate subtraction of h.
+OZ2_HD uint32_t sym_m23(uint32_t w, uint32_t m, uint32_t mu, uint32_t h2) {
+    const uint32_t q = mulhi24((w << 1) + h2, mu);
+    return mad_i24(static_cast<int32_t>(q), -static_cast<int32_t>(m), w);
+}
+
+// Symmetric residue r - h of x for modulus index t >= 1 (m <= 255), r = (x + h) mod m,
+// as a 32-bit value whose low byte is the int8 plane entry, from the octets of
+// u = 2^OFF + x: v = (h - 2^OFF mod m) + sum_j byte_j(u) (2^(8 j) mod m) <= 14 * 255 * 254
+// + 254 < 2^20, one v_dot4 per word.  The chain starts at off - h (mod 2^32), so it
+// yields w = v - h for sym_m23.
 template <Limbs F, typename Tab>
-OZ2_HD uint32_t limb_residue(const uint32_t (&a)[4], Tab tb, int t) {
+OZ2_HD uint32_t sym_residue(const uint32_t (&a)[4], Tab tb, int t) {
     constexpr bool W = F == Limbs::Octets14;
-    uint32_t v = udot4(a[0], tb->c8[t][0], tb->off[W ? 1 : 0][t]);
-    v = udot4(a[1], tb->c8[t][1], v);
+    uint32_t w = udot4(a[0], tb->c8[t][0], tb->offh[W ? 1 : 0][t]);
+    w = udot4(a[1], tb->c8[t][1], w);
     if (W) {
-        v = udot4(a[2], tb->c8[t][2], v);
-        v = udot4(a[3], tb->c8[t][3], v);
+        w = udot4(a[2], tb->c8[t][2], w);
+        w = udot4(a[3], tb->c8[t][3], w);
     }
-    return mod_m23(v, tb->m[t], tb->mu[t]);
+    return sym_m23(w, tb->m[t], tb->mu[t], tb->h2[t]);
 }
 
 // int8 byte of the symmetric residue of x for modulus index t (m = 256 only at t = 0)
 template <Limbs F, typename Tab>
 OZ2_HD uint32_t sym_byte(const uint32_t (&a)[4], Tab tb, int t) {
     if (t == 0) return a[0] & 0xFFu;  // m = 256: x mod 256 = u mod 256
-    return (limb_residue<F>(a, tb, t) - tb->h[t]) & 0xFFu;
+    return sym_residue<F>(a, tb, t) & 0xFFu;
+}
+
+// the low bytes of b0..b3 packed into one word (b0 lowest): two v_perm_b32 (each
+// takes byte 0 of two registers) and an or
+OZ2_HD uint32_t pack_lo_bytes(uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // v_perm_b32 D, S0, S1, sel: selector byte values 0-3 pick bytes of S1, 4-7 bytes of S0, 0x0c gives 0
+    return __builtin_amdgcn_perm(b1, b0, 0x0c0c0400u) | __builtin_amdgcn_perm(b3, b2, 0x04000c0cu);
+#else
+    return (b0 & 0xFFu) | (b1 & 0xFFu) << 8 | (b2 & 0xFFu) << 16 | (b3 & 0xFFu) << 24;
+#endif
 }
 
 // symmetric residue value in [-h, m-1-h] (complex split needs the value, not the byte)
 template <Limbs F, typename Tab>
 OZ2_HD int32_t sym_value(const uint32_t (&a)[4], Tab tb, int t) {
     if (t == 0) return static_cast<int32_t>((a[0] + 127u) & 0xFFu) - 127;
-    return static_cast<int32_t>(limb_residue<F>(a, tb, t)) - static_cast<int32_t>(tb->h[t]);
+    return static_cast<int32_t>(sym_residue<F>(a, tb, t));
 }
 
 // GEMM epilogue: int32 dot product x (as bits) of one k-chunk -> x mod m in [0, m),

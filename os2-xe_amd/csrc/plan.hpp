@@ -87,6 +87,8 @@ struct TabSet {
     uint32_t h[kMaxModuli];                 // (m - 1) / 2
     uint32_t c16[kMaxModuli];               // 2^16 mod m
     uint32_t off[2][kMaxModuli];            // (m - 2^OFF mod m + h) mod m, OFF = 62 (narrow), 104 (wide)
+    uint32_t offh[2][kMaxModuli];           // off - h (mod 2^32): start of the symmetric split's dot chain
+    uint32_t h2[kMaxModuli];                // 2 h
     uint32_t c8[kMaxModuli][4];             // bytes 2^(8 j) mod m, j = 0..15, four to a word (low first)
     uint32_t c30[kMaxModuli];               // m * ceil(2^30 / m) (GEMM residue epilogue offset; 0 for m = 256)
     // CRT (hpp:222-236)

@@ -231,12 +231,17 @@ __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, 
 #pragma unroll
     for (int q = 0; q < Q; ++q) to_limbs<F>(rowzero ? 0 : bits[q], e, false, limbs[q]);
     int8_t* out = a.P + chunk_offset(r, sl.blk, sl.c, a.kp) + sl.q0;
-    for (int t = 0; t < a.s; ++t) {
-        uint32_t w[4] = {0, 0, 0, 0};
+    uint32_t w[4] = {0, 0, 0, 0};
+    // plane 0 has m = 256 in every moduli set: its residue byte is the low byte of u
+#pragma unroll
+    for (int g = 0; g < Q / 4; ++g)
+        w[g] = pack_lo_bytes(limbs[4 * g][0], limbs[4 * g + 1][0], limbs[4 * g + 2][0], limbs[4 * g + 3][0]);
+    store_bytes<Q>(out, w);
+    for (int t = 1; t < a.s; ++t) {
 #pragma unroll
         for (int g = 0; g < Q / 4; ++g)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) w[g] |= sym_byte<F>(limbs[4 * g + q], tb, t) << (8 * q);
+            w[g] = pack_lo_bytes(sym_residue<F>(limbs[4 * g], tb, t), sym_residue<F>(limbs[4 * g + 1], tb, t),
+                                 sym_residue<F>(limbs[4 * g + 2], tb, t), sym_residue<F>(limbs[4 * g + 3], tb, t));
         store_bytes<Q>(out + t * a.plane, w);
     }
 }
@@ -292,6 +297,7 @@ __device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, 
         uint32_t w0[4] = {0, 0, 0, 0}, w1[4] = {0, 0, 0, 0}, w2[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int g = 0; g < Q / 4; ++g) {
+            uint32_t b0[4], b1[4], b2[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int32_t s0 = sym_value<F>(lr[4 * g + q], tb, t);
@@ -300,10 +306,13 @@ __device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, 
                 int32_t ss = s0 + s1;
                 ss = ss > m - 1 - h ? ss - m : ss;
                 ss = ss < -h ? ss + m : ss;
-                w0[g] |= (static_cast<uint32_t>(s0) & 0xFFu) << (8 * q);
-                w1[g] |= (static_cast<uint32_t>(s1) & 0xFFu) << (8 * q);
-                w2[g] |= (static_cast<uint32_t>(ss) & 0xFFu) << (8 * q);
+                b0[q] = static_cast<uint32_t>(s0);
+                b1[q] = static_cast<uint32_t>(s1);
+                b2[q] = static_cast<uint32_t>(ss);
             }
+            w0[g] = pack_lo_bytes(b0[0], b0[1], b0[2], b0[3]);
+            w1[g] = pack_lo_bytes(b1[0], b1[1], b1[2], b1[3]);
+            w2[g] = pack_lo_bytes(b2[0], b2[1], b2[2], b2[3]);
         }
         store_bytes<Q>(out + t * a.plane, w0);
         store_bytes<Q>(out + a.group + t * a.plane, w1);

@@ -85,6 +85,19 @@ int64_t emu_mod_m23_mismatches(int s, int t) {
     return bad;
 }
 
+// symmetric split reduction of modulus t of s over every dot-chain value v < 2^20 + 2^8
+// (w = v - h as the chain leaves it): entries that differ from (v mod m) - h
+int64_t emu_sym_m23_mismatches(int s, int t) {
+    const TabSet* tb = tables() + s;
+    const uint32_t m = tb->m[t], h = tb->h[t];
+    int64_t bad = 0;
+    for (uint32_t v = 0; v < (1u << 20) + 256; ++v) {
+        const int32_t got = static_cast<int32_t>(math::sym_m23(v - h, m, tb->mu[t], tb->h2[t]));
+        bad += got != static_cast<int32_t>(v % m) - static_cast<int32_t>(h);
+    }
+    return bad;
+}
+
 // GEMM epilogue: residues of int32 sums, out[t * n + i] in [0, m_t)
 void emu_acc_residue(const int32_t* x, int64_t n, int s, uint8_t* out) {
     const TabSet* tb = tables() + s;

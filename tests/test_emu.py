@@ -108,6 +108,22 @@ def test_exact_quotient_every_value():
     assert len(seen) == 25
 
 
+def test_symmetric_reduction_every_value():
+    # sym_m23 (real and complex split): from w = v - h (mod 2^32), the chain's start folded
+    # with -h, it gives (v mod m) - h for every dot-chain value v < 2^20 + 2^8 of every
+    # modulus, including v < h where w wraps below zero
+    _emu.emu_sym_m23_mismatches.restype = ctypes.c_int64
+    _emu.emu_sym_m23_mismatches.argtypes = [i32, i32]
+    seen = set()
+    for s in range(2, 25):
+        for t, m in enumerate(O.moduli(s)):
+            if m == 256 or m in seen:
+                continue
+            seen.add(m)
+            assert _emu.emu_sym_m23_mismatches(s, t) == 0, m
+    assert len(seen) == 25
+
+
 @pytest.mark.parametrize("s", [2, 7, 14, 16, 24])
 def test_acc_residue(s, rng):
     # documented domain |x| < 2^30 - 255 (GEMM accumulators of one k-chunk: |x| <= 65408 * 2^14),
