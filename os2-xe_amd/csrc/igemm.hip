@@ -95,9 +95,8 @@ __device__ __forceinline__ void residue_epilogue(const GemmArgs& p, v16i (&acc)[
             for (int j = 0; j < FN; ++j)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    uint32_t w = 0;
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) w |= (static_cast<uint32_t>(acc[i][j][4 * q + b]) & 0xFFu) << (8 * b);
+                    const uint32_t w = pack_lo_bytes(static_cast<uint32_t>(acc[i][j][4 * q]), static_cast<uint32_t>(acc[i][j][4 * q + 1]),
+                                                     static_cast<uint32_t>(acc[i][j][4 * q + 2]), static_cast<uint32_t>(acc[i][j][4 * q + 3]));
                     *reinterpret_cast<uint32_t*>(lds_at(j * 32 + fr, i * 32 + 8 * q + 4 * fc)) = w;
                 }
     } else {
@@ -105,14 +104,15 @@ __device__ __forceinline__ void residue_epilogue(const GemmArgs& p, v16i (&acc)[
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int j = 0; j < FN; ++j)
+            for (int j = 0; j < FN; ++j) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    uint32_t w = 0;
+                    uint32_t r[4];
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) w |= residue_m30(static_cast<uint32_t>(acc[i][j][4 * q + b]), m, mu, c16, c30) << (8 * b);
-                    *reinterpret_cast<uint32_t*>(lds_at(j * 32 + fr, i * 32 + 8 * q + 4 * fc)) = w;
+                    for (int b = 0; b < 4; ++b) r[b] = residue_m30(static_cast<uint32_t>(acc[i][j][4 * q + b]), m, mu, c16, c30);
+                    *reinterpret_cast<uint32_t*>(lds_at(j * 32 + fr, i * 32 + 8 * q + 4 * fc)) = pack_lo_bytes(r[0], r[1], r[2], r[3]);
                 }
+            }
     }
     __syncthreads();
     uint8_t* Rb = p.R + static_cast<int64_t>(plane) * p.plane_r;
