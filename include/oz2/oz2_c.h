@@ -110,7 +110,8 @@ typedef struct {
 } oz2_gemm_desc_t;
 typedef struct oz2_batch_s* oz2_batch;
 /* GRAPH: capture the launch sequence in a hipGraph; SELECT: learned num_moduli for entries with 0;
- * PERSISTENT: persistent GEMM schedule (a fixed grid pulling tiles), else one workgroup per tile;
+ * PERSISTENT: persistent GEMM schedule (a fixed grid pulling tiles) for 128 and 64 tiles, else one
+ *   workgroup per tile (256-wide tiles always take one workgroup per tile);
  * PROBLEM_ORDER: GEMM work in problem order, else largest first */
 enum { OZ2_BATCH_GRAPH = 1, OZ2_BATCH_SELECT = 2, OZ2_BATCH_PERSISTENT = 4, OZ2_BATCH_PROBLEM_ORDER = 8 };
 int oz2_batch_create(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, const int* tiles, int flags,

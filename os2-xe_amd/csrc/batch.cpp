@@ -259,8 +259,8 @@ BatchPlan::~BatchPlan() {
 int BatchPlan::launches() const {
     int n = static_cast<int>(special_.size());
     for (auto* v : {&scale_, &split_, &crt_}) n += static_cast<int>(v->size());
-    // a persistent GEMM launch follows a counter reset
-    return n + (opt_.persistent ? 2 : 1) * static_cast<int>(gemm_.size());
+    for (const k::GroupedLaunch& g : gemm_) n += k::gemm_grouped_launches(g, opt_.persistent);
+    return n;
 }
 
 void BatchPlan::enqueue(hipStream_t st) {

@@ -38,7 +38,7 @@ struct GemmDesc {
 
 // schedule of the grouped INT8 GEMM launches (chosen by measurement, see autotune_batch)
 struct BatchOptions {
-    bool persistent = false;     // fixed grid pulling tiles from a counter, else one workgroup per tile
+    bool persistent = false;     // 128/64 tiles: fixed grid pulling tiles from a counter, else one workgroup per tile
     bool largest_first = true;   // GEMM work listed by decreasing size, else in problem order
 };
 

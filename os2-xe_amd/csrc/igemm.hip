@@ -349,22 +349,27 @@ void prep_gemm(std::vector<GroupedLaunch>& out, const std::vector<GemmJob>& jobs
         if (g.n) out.push_back(std::move(g));
 }
 
+// the persistent loop around a 256-wide tile spills registers (its loop state on top
+// of 256 accumulators per lane), so those groups always take one workgroup per tile
+static bool persistent_form(Tile t, bool persistent) { return persistent && (t == Tile::T128 || t == Tile::T64); }
+
+int gemm_grouped_launches(const GroupedLaunch& g, bool persistent) {
+    if (g.n == 0) return 0;
+    return persistent_form(static_cast<Tile>(g.kind), persistent) ? 2 : 1;  // counter reset + GEMM
+}
+
 void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* counter, int cus, bool persistent) {
     if (g.n == 0) return;
     const Tile t = static_cast<Tile>(g.kind);
     const GemmArgs* items = static_cast<const GemmArgs*>(g.d_items);
-    if (persistent) {
+    if (persistent_form(t, persistent)) {
         hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, counter, 1);
         const uint32_t grid = std::min<uint32_t>(g.blocks, static_cast<uint32_t>(cus * tile_blocks_per_cu(t)));
 #define OZ2_LAUNCH(BM, BN, MINB, NS)                                                                             \
     hipLaunchKernelGGL((igemm_grouped_kernel<BM, BN, MINB, NS>), dim3(grid), dim3(256), 0, st, items, g.d_start, \
                        g.d_owner, g.blocks, counter)
-        switch (t) {
-            case Tile::T256: OZ2_LAUNCH(256, 256, 1, 4); break;
-            case Tile::T128: OZ2_LAUNCH(128, 128, 2, 4); break;
-            case Tile::T64: OZ2_LAUNCH(64, 64, 4, 4); break;
-            case Tile::T256x128: OZ2_LAUNCH(256, 128, 2, 3); break;
-        }
+        if (t == Tile::T128) OZ2_LAUNCH(128, 128, 2, 4);
+        else OZ2_LAUNCH(64, 64, 4, 4);
 #undef OZ2_LAUNCH
         return;
     }

@@ -131,9 +131,10 @@ void prep_split(std::vector<GroupedLaunch>& out, const std::vector<SplitArgs>& j
 void prep_gemm(std::vector<GroupedLaunch>& out, const std::vector<GemmJob>& jobs);
 void prep_crt(std::vector<GroupedLaunch>& out, const std::vector<CrtArgs>& jobs);
 bool launch_scale_split_grouped(hipStream_t st, const GroupedLaunch& g);
-// persistent: min(blocks, cus x blocks per CU) workgroups pull tiles from *counter (reset first);
-// else one workgroup per tile (counter unused)
+// persistent (128 and 64 tiles): min(blocks, cus x blocks per CU) workgroups pull tiles from
+// *counter (reset first); else, and always for 256-wide tiles, one workgroup per tile (counter unused)
 void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* counter, int cus, bool persistent);
+int gemm_grouped_launches(const GroupedLaunch& g, bool persistent);  // kernels launch_gemm_grouped enqueues
 void launch_crt_grouped(hipStream_t st, const GroupedLaunch& g);
 
 }  // namespace k
