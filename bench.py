@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--vendor-int8", action="store_true",
                     help="also time the vendor INT8 GEMM (hipBLASLt through torch._int_mm) on s GEMMs of the same "
                          "shape, int32 output, as the rate this GPU reaches with a library kernel")
+    ap.add_argument("--vendor-fp8", action="store_true",
+                    help="also time the vendor FP8 GEMM (hipBLASLt through torch._scaled_mm) on s GEMMs of the same "
+                         "shape: the same nominal peak as INT8, for comparing sustained rates and clocks")
     ap.add_argument("--workload", default="dgemm8192",
                     choices=["dgemm8192", "zgemm4096", "wide4000", "ksplit", "batch"],
                     help="dgemm8192 (default line), or one of the other measured configurations")
@@ -391,6 +394,41 @@ def vendor_int8(torch, m, n, k, s, reps=5):
     return out
 
 
+def vendor_fp8(torch, m, n, k, s, reps=5):
+    """hipBLASLt float8_e4m3fn x float8_e4m3fn -> bf16 (torch._scaled_mm, unit scales) on s GEMMs of m x n x k,
+    op(B) = T, best of reps: the same nominal MFMA peak as INT8 (5 POP/s dense), for comparing the rate this GPU
+    sustains with each element type."""
+    out = {"library": "hipBLASLt (torch._scaled_mm) float8_e4m3fn, bf16 output", "gemms": s}
+    try:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(4)
+        A = [(torch.randn(m, k, device="cuda", generator=g) * 0.1).to(torch.float8_e4m3fn) for _ in range(2)]
+        B = [(torch.randn(n, k, device="cuda", generator=g) * 0.1).to(torch.float8_e4m3fn).t() for _ in range(2)]
+        one = torch.tensor(1.0, device="cuda")
+
+        def run(i):
+            torch._scaled_mm(A[i & 1], B[i & 1], scale_a=one, scale_b=one, out_dtype=torch.bfloat16)
+
+        run(0)
+        torch.cuda.synchronize()
+    except (RuntimeError, AttributeError) as e:
+        out["error"] = str(e)[:200]
+        return out
+    best = float("inf")
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for i in range(s):
+            run(i)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    out["NT_ms"] = round(best, 4)
+    out["NT_tflops"] = round(2.0 * m * n * k * s / best / 1e9, 1)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -512,6 +550,8 @@ def main():
         out["sweep_tflops"] = sweep
     if args.vendor_int8:
         out["roofline"]["vendor_int8"] = vendor_int8(torch, m, n, k, s)
+    if args.vendor_fp8:
+        out["roofline"]["vendor_fp8"] = vendor_fp8(torch, m, n, k, s)
     if not args.no_batch:
         del dA, dB, dC, dC0
         torch.cuda.empty_cache()
