@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--vendor-fp8", action="store_true",
                     help="also time the vendor FP8 GEMM (hipBLASLt through torch._scaled_mm) on s GEMMs of the same "
                          "shape: the same nominal peak as INT8, for comparing sustained rates and clocks")
+    ap.add_argument("--mfma-rate", action="store_true",
+                    help="also time a register-only INT8 MFMA loop on every CU: the rate the matrix cores "
+                         "sustain with no memory traffic, the practical ceiling for the GEMM")
     ap.add_argument("--workload", default="dgemm8192",
                     choices=["dgemm8192", "zgemm4096", "wide4000", "ksplit", "batch"],
                     help="dgemm8192 (default line), or one of the other measured configurations")
@@ -429,6 +432,30 @@ def vendor_fp8(torch, m, n, k, s, reps=5):
     return out
 
 
+def mfma_rate(torch, reps=3):
+    """sustained INT8 MFMA rate with no memory traffic (liboz2_check.so oz2chk_mfma_rate: 16 independent
+    32x32x32 chains per wave, one wave per SIMD, 4 workgroups per CU over the launch), best of reps"""
+    import ctypes
+    from oz2 import check
+    L = check.lib()
+    L.oz2chk_mfma_rate.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    blocks, iters = 4 * torch.cuda.get_device_properties(0).multi_processor_count, 4096
+    out = torch.empty(blocks * 256, dtype=torch.int32, device="cuda")
+    best = float("inf")
+    for _ in range(reps + 1):  # the first launch warms the clock up
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        if L.oz2chk_mfma_rate(None, blocks, iters, out.data_ptr()) != 0:
+            raise RuntimeError("oz2: MFMA rate kernel failed")
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    ops = 2.0 * 32768 * 16 * iters * 4 * blocks
+    return {"kernel": "register-only v_mfma_i32_32x32x32_i8 loop", "ms": round(best, 4),
+            "tops": round(ops / best / 1e9, 1), "frac_of_peak": round(ops / best / 1e9 / INT8_PEAK_TOPS, 4)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -552,6 +579,8 @@ def main():
         out["roofline"]["vendor_int8"] = vendor_int8(torch, m, n, k, s)
     if args.vendor_fp8:
         out["roofline"]["vendor_fp8"] = vendor_fp8(torch, m, n, k, s)
+    if args.mfma_rate:
+        out["roofline"]["mfma_register_loop"] = mfma_rate(torch)
     if not args.no_batch:
         del dA, dB, dC, dC0
         torch.cuda.empty_cache()
