@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per INT8 GEMM launch measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE passes,
 # each scaled by its reading for a 1 GiB copy kernel): profiles/dgemm8192_s16_hbm_traffic.txt
-MEASURED_GEMM_TRAFFIC = {(8192, 8192, 8192, 16): 14.64e9}
+MEASURED_GEMM_TRAFFIC = {(8192, 8192, 8192, 16): 14.71e9}
 
 
 def parse():
