@@ -43,3 +43,15 @@ def test_throughput_rows():
         assert err < (1e-5 if r["s"] == 8 else 1e-14), ln
         assert " dgemm " in ln and r["dgemm"] > 0 and r["best"] <= r["median"]
     assert "vs GPU DGEMM" in text
+
+
+def test_mfma_rate_harness():
+    # bench.py --mfma-rate: the register-only MFMA loop runs on every CU and reports a rate
+    # below the nominal peak (a loop that did less work than counted would exceed it)
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location("bench_main", os.path.join(os.path.dirname(__file__), "..", "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    r = bench.mfma_rate(torch, reps=1)
+    assert 1000.0 < r["tops"] <= 1.02 * bench.INT8_PEAK_TOPS, r
