@@ -4,12 +4,15 @@ alpha/beta, padded leading dimensions, and random tuning (per-plane launches,
 plane padding, power-of-two candidates, forced GEMM tiles) and small workspace
 limits that force multi-tile runs.  Every C must be bit-identical to the oracle on
 the same inputs (the reference's algorithm restated; tests/test_ozaki2.cpp:95-194
-runs one fixed case per behaviour, this covers their combinations)."""
+runs one fixed case per behaviour, this covers their combinations).  Random batches
+(oz2_batch_*) of such problems, with random tiles, schedules, order and graph use,
+must reproduce oz2_dgemm bit for bit on every replay."""
 import numpy as np
 import pytest
+import torch
 
 import oz2
-from gpu_cases import complex_case, log, real_case
+from gpu_cases import complex_case, log, real_case, to_dev
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -51,5 +54,46 @@ def test_random_cases_match_oracle(seed):
             except oz2.OZ2RuntimeError as e:
                 # a workspace limit below one tile's need is reported, not worked around
                 assert limit and "not enough device memory" in str(e), e
+    finally:
+        oz.close()
+
+
+@pytest.mark.parametrize("seed", [404, 505])
+def test_random_batches_match_single_calls(seed):
+    # batches of 4-24 random problems (every op, s, alpha/beta, k including 0 and
+    # k-chunked lengths), random per-problem tiles, schedule, order and graph use,
+    # replayed twice: every problem's C bit-identical to oz2_dgemm with the same s
+    from test_gpu_batch import build, check_same, desc, reference
+    rng = np.random.default_rng(seed)
+    oz = oz2.Ozaki2()
+    try:
+        for c in range(6):
+            problems = []
+            for _ in range(int(rng.integers(4, 25))):
+                k = int(rng.choice([0, int(rng.integers(1, 700)), int(rng.integers(700, 4000)), 70000 + int(rng.integers(0, 9))]))
+                s = int(rng.integers(3, 25))
+                while O.precision_bits(max(k, 1), s) < 0:
+                    s = int(rng.integers(3, 25))
+                m = int(rng.integers(1, 300)) if k < 70000 else int(rng.integers(1, 40))
+                n = int(rng.integers(1, 300)) if k < 70000 else int(rng.integers(1, 40))
+                problems.append((str(rng.choice(["N", "T"])), str(rng.choice(["N", "T"])), m, n, k, s,
+                                 str(rng.choice(["u11", "uniform", "phi", "wide"])),
+                                 float(rng.choice([1.0, -1.5, 0.0])), float(rng.choice([0.0, 0.5, -2.0]))))
+            ps = build(problems, seed=int(rng.integers(1 << 30)))
+            ref = [reference(oz, p) for p in ps]
+            tiles = [int(rng.choice([0, 256, 128, 64, 256128])) for _ in ps] if rng.random() < 0.5 else None
+            persistent, largest_first, graph = bool(rng.random() < 0.5), bool(rng.random() < 0.5), bool(rng.random() < 0.7)
+            log(f"random batch {seed}/{c}: {len(ps)} problems tiles={tiles} persistent={persistent} "
+                f"largest_first={largest_first} graph={graph}")
+            Cs = [to_dev(p["C0"].copy()) for p in ps]
+            b = oz2.Batch(oz, [desc(p, C) for p, C in zip(ps, Cs)], tiles=tiles, persistent=persistent,
+                          largest_first=largest_first, graph=graph)
+            for _ in range(2):
+                for C, p in zip(Cs, ps):
+                    C.copy_(to_dev(p["C0"]))
+                b.run()
+                torch.cuda.synchronize()
+                check_same(ref, Cs, f"random batch {seed}/{c}")
+            b.close()
     finally:
         oz.close()
