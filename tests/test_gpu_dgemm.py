@@ -220,3 +220,10 @@ def test_stream_and_events(oz):
         assert np.array_equal(bits(dC.cpu().numpy()), bits(ref))
     finally:
         h.close()
+
+
+def test_cpu_baseline_configuration(oz):
+    # the reference's CPU-runnable configuration: DGEMM 1024^3 N/N, alpha = 1, beta = 0,
+    # U[-1,1), s = 14 (tk = 49), every entry against the oracle and the rigorous bound
+    r = real_case(oz, "N", "N", 1024, 1024, 1024, 14, "u11")
+    assert r["bound"] <= 1.0
