@@ -92,3 +92,9 @@ def complex_case(oz, ta, tb, m, n, k, s, dist, alpha=1 + 0j, beta=0j, ld_extra=0
         res["bound"] = r
         assert r <= 1.0, f"err/bound = {r}"
     return res
+
+
+def fma(a, b, c):
+    """correctly rounded a*b + c in double precision (Python's Fraction -> float rounds to nearest even)"""
+    from fractions import Fraction
+    return float(Fraction(a) * Fraction(b) + Fraction(c))

@@ -227,3 +227,27 @@ def test_cpu_baseline_configuration(oz):
     # U[-1,1), s = 14 (tk = 49), every entry against the oracle and the rigorous bound
     r = real_case(oz, "N", "N", 1024, 1024, 1024, 14, "u11")
     assert r["bound"] <= 1.0
+
+
+@pytest.mark.slow
+def test_full_size_configuration_sampled(oz):
+    # the benchmark configuration at full size: DGEMM 8192^3 N/N, s = 16, alpha = 1.5,
+    # beta = 0.5, C0 U[-1,1); sampled entries against the exact closed form with the
+    # library's epilogue fma(alpha, v, beta*c) applied exactly on the host
+    from gpu_cases import fma
+    m = n = k = 8192
+    rng = np.random.default_rng(17)
+    A = rng.uniform(-1, 1, m * k)
+    B = rng.uniform(-1, 1, k * n)
+    C0 = rng.uniform(-1, 1, m * n)
+    log("case full-size dgemm NN 8192^3 s=16 alpha=1.5 beta=0.5")
+    dC = to_dev(C0)
+    oz.gemm("N", "N", m, n, k, 1.5, to_dev(A), m, to_dev(B), k, 0.5, dC, m, 16)
+    C = dC.cpu().numpy()
+    rows = rng.integers(0, m, 256)
+    cols = rng.integers(0, n, 256)
+    rows[:4] = [0, m - 1, 0, m - 1]
+    cols[:4] = [0, 0, n - 1, n - 1]
+    v = O.gemm_entries("N", "N", m, n, k, A, m, B, k, 16, rows, cols)
+    want = np.array([fma(1.5, float(x), 0.5 * C0[i + j * m]) for x, i, j in zip(v, rows, cols)])
+    assert np.array_equal(bits(C[rows + cols * m]), bits(want))

@@ -108,3 +108,37 @@ def test_large_short_k_rectangular_tile(oz):
 def test_large_wide_split(oz):
     # s = 24: tk > 62, the wide complex split (half-chunk slots), long k: 256 x 256 tiles
     _sampled_complex(oz, "T", "C", 600, 700, 3000, 24, "phi", ld_extra=1)
+
+
+@pytest.mark.slow
+def test_full_size_configuration_sampled(oz):
+    # the benchmark configuration at full size: ZGEMM 4096^3, op(A) = C, op(B) = N, s = 16,
+    # alpha = (1, 0.5), beta = (0.25, -1), lda = m + 7, ldc = m + 3; sampled entries against
+    # the exact closed form with the library's complex epilogue applied exactly on the host
+    import torch
+    from gpu_cases import fma
+    m = n = k = 4096
+    lda, ldb, ldc = m + 7, k, m + 3
+    ar_, ai_, br_, bi_ = 1.0, 0.5, 0.25, -1.0
+    rng = np.random.default_rng(19)
+    A = rng.uniform(-1, 1, 2 * lda * m)  # op(A) = C: A is k x m, interleaved (re, im)
+    B = rng.uniform(-1, 1, 2 * ldb * n)
+    C0 = rng.uniform(-1, 1, 2 * ldc * n)
+    log("case full-size zgemm CN 4096^3 s=16 alpha=(1,0.5) beta=(0.25,-1) lda=m+7 ldc=m+3")
+    dC = to_dev(C0)
+    oz.zgemm("C", "N", m, n, k, complex(ar_, ai_), to_dev(A), lda, to_dev(B), ldb, complex(br_, bi_), dC, ldc, 16)
+    torch.cuda.synchronize()
+    C = dC.cpu().numpy()
+    rows = rng.integers(0, m, 192)
+    cols = rng.integers(0, n, 192)
+    rows[:4] = [0, m - 1, 0, m - 1]
+    cols[:4] = [0, 0, n - 1, n - 1]
+    v = O.gemm_entries("C", "N", m, n, k, A, lda, B, ldb, 16, rows, cols, cplx=True).reshape(-1, 2)
+    want = []
+    for (vr, vi), i, j in zip(v, rows, cols):
+        cr, ci = C0[2 * (i + j * ldc)], C0[2 * (i + j * ldc) + 1]
+        re = fma(ar_, float(vr), -(ai_ * float(vi))) + fma(br_, cr, -(bi_ * ci))
+        im = fma(ar_, float(vi), ai_ * float(vr)) + fma(br_, ci, bi_ * cr)
+        want.append((re, im))
+    got = np.stack([C[2 * (rows + cols * ldc)], C[2 * (rows + cols * ldc) + 1]], axis=1)
+    assert np.array_equal(bits(got), bits(np.array(want)))
