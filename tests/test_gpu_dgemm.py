@@ -251,3 +251,13 @@ def test_full_size_configuration_sampled(oz):
     v = O.gemm_entries("N", "N", m, n, k, A, m, B, k, 16, rows, cols)
     want = np.array([fma(1.5, float(x), 0.5 * C0[i + j * m]) for x, i, j in zip(v, rows, cols)])
     assert np.array_equal(bits(C[rows + cols * m]), bits(want))
+
+
+@pytest.mark.slow
+def test_selector_configurations_sampled(oz):
+    # the selector benchmarks at full size, at the s the selector picks there (24): the wide-exponent
+    # DGEMM 4000x4000 (k = 8000) and the k-split DGEMM 2048x2048 (k = 200000, four k-chunks of at most
+    # 65408), entries (1 + U) * (+-1) * 2^e with e in [-20, 20]; sampled entries against the closed form
+    _sampled_large(oz, "N", "N", 4000, 4000, 8000, 24, "wide", seed=23, count=256)
+    _sampled_large(oz, "N", "N", 2048, 2048, 200000, 24, "wide", seed=29, count=96)
+    assert oz.last_tiling().k_chunks == 4
