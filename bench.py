@@ -26,6 +26,7 @@ for _p in (ROOT, os.path.join(ROOT, "os2-xe_amd")):
         sys.path.insert(0, _p)
 
 INT8_PEAK_TOPS = 5000.0  # MI355X dense INT8 MFMA, 5.0 POP/s
+HBM_PEAK_GBS = 8000.0    # MI355X HBM3E, 8 TB/s
 HBM_PEAK_GBS = 8000.0
 
 
@@ -201,6 +202,28 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
     return out
 
 
+def hbm_stages(m, n, k, s, cplx, beta_nonzero, stage):
+    """algorithmic HBM bytes of the bandwidth-bound stages and their achieved GB/s from the stage times:
+    scale reads the operands; the split reads them and writes s (3s complex) int8 planes padded to the
+    256-row GEMM tile and k to 128; the CRT reads s (3s) residue bytes per entry and writes C (reading it
+    too when beta != 0)"""
+    def ru(x, a):
+        return (x + a - 1) // a * a
+    w, g = (2, 3) if cplx else (1, 1)
+    kp = ru(k, 128)
+    elems = m * k + k * n
+    nbytes = {"scale": 8 * w * elems,
+              "split": 8 * w * elems + s * g * (ru(m, 256) + ru(n, 256)) * kp,
+              "crt": s * g * m * n + 8 * w * m * n * (2 if beta_nonzero else 1)}
+    out = {}
+    for name, b in nbytes.items():
+        ms = stage.get(f"{name}_ms", 0.0)
+        gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        out[name] = {"bytes": b, "ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["peak_GB/s"] = HBM_PEAK_GBS
+    return out
+
+
 def zgemm_line(args, torch):
     """ZGEMM 4096^3, op(A)=C, op(B)=N, s=16, alpha=(1,0.5), beta=(0.25,-1), lda=m+7, ldc=m+3 (3M form)"""
     import oz2
@@ -241,6 +264,7 @@ def zgemm_line(args, torch):
                         "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
                         "kernel": "igemm_residue_kernel (2*m*n*k*3s int8 ops per launch)", "gemm_ms": round(gemm_ms, 4)},
            "stage_ms": {kk: round(v / max(1, calls), 4) for kk, v in st.as_dict().items()}}
+    out["hbm_stages"] = hbm_stages(m, n, k, s, True, True, out["stage_ms"])
     # accuracy with alpha=1, beta=0 against double-double (complex)
     dC1 = torch.empty(2 * ldc * n, dtype=torch.float64, device="cuda")
     oz.zgemm("C", "N", m, n, k, 1.0, dA, lda, dB, ldb, 0.0, dC1, ldc, s)
@@ -550,6 +574,7 @@ def main():
                      "kernel": "igemm_residue_kernel (2*m*n*k*s int8 ops per launch)",
                      "gemm_ms": round(gemm_ms, 4)},
         "stage_ms": {kk: round(v, 4) for kk, v in stage.items()},
+        "hbm_stages": hbm_stages(m, n, k, s, False, True, stage),
         "int8_rate_end_to_end": round(int8_ops / (ms * 1e-3) / 1e12, 1),
         "reference_b60_tflops": 9.3,
     }
