@@ -65,7 +65,8 @@ def real_case(oz, ta, tb, m, n, k, s, dist, alpha=1.0, beta=0.0, lda_extra=0, ld
 
 
 def complex_case(oz, ta, tb, m, n, k, s, dist, alpha=1 + 0j, beta=0j, ld_extra=0, phi=1.0, seed=3,
-                 check_oracle=True, check_bound=True):
+                 check_oracle=True, check_bound=True, c_shift=0):
+    """c_shift: C starts that many doubles into its device buffer (1 = only 8-byte aligned)"""
     rng = np.random.default_rng(seed)
     ar, ac = (m, k) if ta == "N" else (k, m)
     br, bc = (k, n) if tb == "N" else (n, k)
@@ -74,10 +75,12 @@ def complex_case(oz, ta, tb, m, n, k, s, dist, alpha=1 + 0j, beta=0j, ld_extra=0
     B = make_cmat(br, bc, ldb, dist, rng, phi).reshape(-1, order="F").view(np.float64).copy()
     C0 = rng.uniform(-1, 1, 2 * ldc * n)
     log(f"case zgemm {ta}{tb} m={m} n={n} k={k} s={s} {dist} alpha={alpha} beta={beta}")
-    dC = to_dev(C0.copy())
+    dbuf = to_dev(np.concatenate([np.full(c_shift, 7.0), C0]))
+    dC = dbuf[c_shift:]
     oz.zgemm(ta, tb, m, n, k, alpha, to_dev(A), lda, to_dev(B), ldb, beta, dC, ldc, s)
     torch.cuda.synchronize()
     C = dC.cpu().numpy()
+    assert np.all(dbuf[:c_shift].cpu().numpy() == 7.0)
     res = {"C": C}
     for j in range(n):
         assert np.array_equal(C[2 * (j * ldc + m):2 * (j + 1) * ldc], C0[2 * (j * ldc + m):2 * (j + 1) * ldc])

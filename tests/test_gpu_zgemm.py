@@ -30,6 +30,14 @@ def test_complex_alpha_beta(oz):
     complex_case(oz, "N", "C", 64, 48, 300, 16, "uniform", alpha=0.5 - 1.25j, beta=0.25 + 2.0j, ld_extra=1)
 
 
+def test_c_only_8_byte_aligned(oz):
+    # C one double past a 16-byte boundary, with and without beta: std::complex<double>
+    # arrays are 16-byte aligned, a pointer into a double array need not be, and the
+    # result must not depend on it
+    complex_case(oz, "N", "N", 33, 21, 77, 12, "phi", alpha=1.5 - 0.5j, beta=-0.75 + 0.25j, ld_extra=1, c_shift=1)
+    complex_case(oz, "C", "T", 40, 9, 130, 16, "uniform", c_shift=1)
+
+
 @pytest.mark.parametrize("s", [4, 8, 12, 16, 20, 24])
 def test_accuracy_vs_s(oz, s):
     complex_case(oz, "N", "N", 128, 128, 1024, s, "phi")
