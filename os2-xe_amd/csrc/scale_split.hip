@@ -52,7 +52,18 @@ __device__ __forceinline__ void scale_kcontig_body(const ScaleKcItem& it, uint32
     if ((reinterpret_cast<uintptr_t>(row) & 15) == 0) {
         const int64_t n2 = ndbl >> 1;
         const double2* row2 = reinterpret_cast<const double2*>(row);
-        for (int64_t i = lane; i < n2; i += 64) {
+        constexpr int U = 8;  // loads in flight per step, all at i < n2
+        int64_t i = lane;
+        for (; i + 64 * (U - 1) < n2; i += 64 * U) {
+            double2 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = row2[i + 64 * u];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                mx = max(mx, max(static_cast<uint64_t>(__double_as_longlong(v[u].x) & 0x7FFFFFFFFFFFFFFFll),
+                                 static_cast<uint64_t>(__double_as_longlong(v[u].y) & 0x7FFFFFFFFFFFFFFFll)));
+        }
+        for (; i < n2; i += 64) {
             const double2 v = row2[i];
             const uint64_t a = __double_as_longlong(v.x) & 0x7FFFFFFFFFFFFFFFll;
             const uint64_t b = __double_as_longlong(v.y) & 0x7FFFFFFFFFFFFFFFll;
@@ -85,14 +96,34 @@ __device__ __forceinline__ void scale_kstrided_body(const ScaleKsItem& it, uint3
     const int64_t ldx = it.ldx;
     uint64_t mx = 0;
     if (r < it.len) {
+        // U loads in flight per step (all of them at k < p1), then the remainder one by one:
+        // a single load per step leaves each wave waiting out the full memory latency
+        constexpr int U = 8;
+        int64_t p = p0 + kl;
         if (it.cplx) {
-            for (int64_t p = p0 + kl; p < p1; p += 4) {
+            for (; p + 4 * (U - 1) < p1; p += 4 * U) {
+                double2 v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const double2*>(X + 2 * (r + (p + 4 * u) * ldx));
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    mx = max(mx, max(static_cast<uint64_t>(__double_as_longlong(v[u].x) & 0x7FFFFFFFFFFFFFFFll),
+                                     static_cast<uint64_t>(__double_as_longlong(v[u].y) & 0x7FFFFFFFFFFFFFFFll)));
+            }
+            for (; p < p1; p += 4) {
                 const double2 v = *reinterpret_cast<const double2*>(X + 2 * (r + p * ldx));
                 mx = max(mx, max(static_cast<uint64_t>(__double_as_longlong(v.x) & 0x7FFFFFFFFFFFFFFFll),
                                  static_cast<uint64_t>(__double_as_longlong(v.y) & 0x7FFFFFFFFFFFFFFFll)));
             }
         } else {
-            for (int64_t p = p0 + kl; p < p1; p += 4) mx = max(mx, absbits(X + r + p * ldx));
+            for (; p + 4 * (U - 1) < p1; p += 4 * U) {
+                uint64_t v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) v[u] = absbits(X + r + (p + 4 * u) * ldx);
+#pragma unroll
+                for (int u = 0; u < U; ++u) mx = max(mx, v[u]);
+            }
+            for (; p < p1; p += 4) mx = max(mx, absbits(X + r + p * ldx));
         }
     }
     red[kl][rl] = mx;
@@ -114,7 +145,14 @@ __device__ __forceinline__ void scale_finalize_body(const ScaleFinItem& it, uint
     const int64_t r = static_cast<int64_t>(bx) * 256 + threadIdx.x;
     if (r >= it.len) return;
     uint64_t mx = 0;
-    for (int b = 0; b < it.nkb; ++b) mx = max(mx, it.partial[static_cast<int64_t>(b) * it.len + r]);
+    int b = 0;
+    for (; b + 3 < it.nkb; b += 4) {  // four loads in flight, all at b < nkb
+        uint64_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = it.partial[static_cast<int64_t>(b + u) * it.len + r];
+        mx = max(mx, max(max(v[0], v[1]), max(v[2], v[3])));
+    }
+    for (; b < it.nkb; ++b) mx = max(mx, it.partial[static_cast<int64_t>(b) * it.len + r]);
     it.ex[r] = exponent_from_maxbits(mx, it.tk);
 }
 
