@@ -70,8 +70,17 @@ __device__ __forceinline__ void scale_kcontig_body(const ScaleKcItem& it, uint32
             mx = max(mx, max(a, b));
         }
         if ((ndbl & 1) && lane == 0) mx = max(mx, absbits(row + ndbl - 1));
-    } else {
-        for (int64_t i = lane; i < ndbl; i += 64) mx = max(mx, absbits(row + i));
+    } else {  // a row that is only 8-byte aligned (odd leading dimension): single doubles
+        constexpr int U = 8;
+        int64_t i = lane;
+        for (; i + 64 * (U - 1) < ndbl; i += 64 * U) {
+            uint64_t v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = absbits(row + i + 64 * u);
+#pragma unroll
+            for (int u = 0; u < U; ++u) mx = max(mx, v[u]);
+        }
+        for (; i < ndbl; i += 64) mx = max(mx, absbits(row + i));
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) mx = max(mx, shfl_xor_u64(mx, o));

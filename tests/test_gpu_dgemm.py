@@ -32,6 +32,14 @@ def test_shapes_and_ops(oz, ta, tb):
     real_case(oz, ta, tb, 77, 65, 300, 14, "uniform", alpha=-1.5, beta=0.25, ldc_extra=2)
 
 
+@pytest.mark.parametrize("ta,tb", OPS)
+def test_odd_leading_dimension_long_rows(oz, ta, tb):
+    # lda = ldb = k + 1 (or m/n + 1) odd: every other k-contiguous row is only 8-byte aligned, and
+    # rows of 1000 and 3001 doubles run the scale's many-loads-in-flight loop and its remainder
+    real_case(oz, ta, tb, 40, 33, 1000, 14, "phi", lda_extra=1, ldc_extra=1)
+    real_case(oz, ta, tb, 9, 6, 3001, 16, "uniform", beta=0.5, lda_extra=2)
+
+
 def test_accuracy_vs_s(oz):
     # test:457-463: every s that the plan accepts
     for s in range(2, 25):
