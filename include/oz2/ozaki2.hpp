@@ -24,7 +24,10 @@ inline constexpr int kMaxLimbs = 6;
 inline constexpr int kMaxScaleBits = 100;
 inline constexpr int kNarrowScaleBits = 62;
 inline constexpr int kNonFinite = std::numeric_limits<int>::min();
-inline constexpr int64_t kMaxKChunk = 131071;
+// Longest k-chunk of one INT8 GEMM pass in this library (the reference's kMaxKChunk is 131071,
+// hpp:32): chunks of at most 65408 keep every int32 sum below 2^30, and the chunks of one product
+// have equal length, so last_tiling().k_chunks = ceil(round_up(k, 128) / 65408).
+inline constexpr int64_t kMaxKChunk = 65408;
 
 enum class Trans { N, T, C };
 

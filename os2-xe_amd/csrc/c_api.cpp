@@ -239,6 +239,12 @@ int oz2_workspace_query(oz2_handle h, int ta, int tb, int64_t m, int64_t n, int6
             a = a == oz2::Op::C ? oz2::Op::T : a;
             b = b == oz2::Op::C ? oz2::Op::T : b;
         }
+        if (m < 0 || n < 0 || k < 0) throw std::invalid_argument("oz2: negative dimension");
+        // m, n or k = 0 only scales C (as oz2_dgemm does): no plan and no workspace
+        if (m == 0 || n == 0 || k == 0) {
+            *bytes = 0;
+            return;
+        }
         oz2::make_plan(num_moduli, cplx ? 2 * k : k);
         *bytes = h->eng.workspace_for(a, b, m, n, k, num_moduli, cplx != 0);
     });

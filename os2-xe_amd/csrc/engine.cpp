@@ -56,16 +56,31 @@ Engine::Engine(hipStream_t stream) : stream_(stream) {
         hip_check(e, "table upload");
     }
     int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
-                                                 hipSuccess && cus > 0)
-        cus_ = cus;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        dev_ = dev;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0) cus_ = cus;
+    }
+    if (hipEventCreateWithFlags(&switch_ev_, hipEventDisableTiming) != hipSuccess) {
+        (void)hipFree(dtabs_);
+        throw std::runtime_error("oz2: event create failed");
+    }
 }
 
 Engine::~Engine() {
     (void)hipStreamSynchronize(stream_);
     release_workspace();
     for (auto e : events_) (void)hipEventDestroy(e);
+    if (switch_ev_) (void)hipEventDestroy(switch_ev_);
     if (dtabs_) (void)hipFree(dtabs_);
+}
+
+void Engine::set_stream(hipStream_t s) {
+    if (s == stream_) return;
+    // the workspace (and the selector's buffers) may still be in use by work on the old
+    // stream: the new stream waits for all of it before anything else runs there
+    hip_check(hipEventRecord(switch_ev_, stream_), "event record");
+    hip_check(hipStreamWaitEvent(s, switch_ev_, 0), "stream wait");
+    stream_ = s;
 }
 
 void Engine::release_workspace() {
@@ -210,13 +225,14 @@ Engine::Layout Engine::layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int
 }
 
 size_t Engine::workspace_for(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx) {
+    // a call with an empty C, k = 0 or alpha = 0 only scales C and needs no workspace
+    if (m == 0 || n == 0 || k == 0) return 0;
     const int64_t kch = k_chunks(round_up(k, kKStep));
-    choose_tiling(ta, tb, m, n, k, s, cplx, kch);
-    return tiling_.workspace_bytes;
+    return choose_tiling(ta, tb, m, n, k, s, cplx, kch, budget()).workspace_bytes;
 }
 
-void Engine::choose_tiling(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch) {
-    const size_t limit = budget();
+TileInfo Engine::choose_tiling(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch,
+                               size_t limit) const {
     const double g = cplx ? 3.0 : 1.0;
     auto candidates = [&](int64_t x) {
         std::vector<int64_t> c{x};
@@ -255,7 +271,7 @@ void Engine::choose_tiling(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s,
     if (!std::isfinite(best))
         throw std::runtime_error("oz2: not enough device memory for workspace (limit " + std::to_string(limit >> 20) +
                                  " MiB)");
-    tiling_ = bt;
+    return bt;
 }
 
 void Engine::gemm(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
@@ -291,7 +307,7 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
     const int64_t kp = round_up(k, kKStep);
     const int64_t kch = k_chunks(kp);
     const int64_t klen = k_chunk_len(kp);
-    choose_tiling(ta, tb, m, n, k, s, cplx, kch);
+    tiling_ = choose_tiling(ta, tb, m, n, k, s, cplx, kch, budget());
     const int64_t mb = tiling_.mb, nb = tiling_.nb;
     const Layout L = layout(ta, tb, m, n, k, s, cplx, kch, mb, nb);
     tile_ = L.tile;

@@ -66,7 +66,9 @@ public:
     void profile_begin();
     StageTimes profile_end(int64_t* calls);
     hipStream_t stream() const { return stream_; }
-    void set_stream(hipStream_t s) { stream_ = s; }
+    // later work goes to `s`, ordered after everything already queued on the old stream
+    void set_stream(hipStream_t s);
+    int device() const { return dev_; }
 
     // Workspace a call would need (no allocation); used to pre-size before graph capture.
     size_t workspace_for(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx);
@@ -86,10 +88,13 @@ private:
     struct Layout;
     Layout layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch, int64_t mb,
                   int64_t nb) const;
-    void choose_tiling(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch);
+    TileInfo choose_tiling(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch,
+                           size_t limit) const;
     void ensure_workspace(size_t bytes);
 
     hipStream_t stream_;
+    int dev_ = 0;
+    hipEvent_t switch_ev_ = nullptr;  // orders a new stream after the old one (set_stream)
     TabSet* dtabs_ = nullptr;  // device copy of make_tabsets(), s = 0..24
     Tuning tune_{};
     bool profiling_ = true;
