@@ -2,7 +2,6 @@
 #include "plan.hpp"
 
 #include <algorithm>
-#include <cmath>
 #include <cstring>
 
 namespace oz2 {
@@ -34,19 +33,15 @@ void Wide::mul(uint32_t f) {
     if (c) throw std::overflow_error("oz2: integer overflow in plan arithmetic");
 }
 
-uint32_t Wide::divmod(uint32_t d) {
+uint64_t Wide::divmod(uint64_t d) {
+    // schoolbook division by one 64-bit digit: the running remainder stays below d
     unsigned __int128 r = 0;
     for (int i = N - 1; i >= 0; --i) {
         const unsigned __int128 cur = (r << 64) | v[i];
         v[i] = static_cast<uint64_t>(cur / d);
         r = cur % d;
     }
-    return static_cast<uint32_t>(r);
-}
-
-uint32_t Wide::mod(uint32_t d) const {
-    Wide t = *this;
-    return t.divmod(d);
+    return static_cast<uint64_t>(r);
 }
 
 int Wide::bits() const {
@@ -55,47 +50,22 @@ int Wide::bits() const {
     return 0;
 }
 
-bool Wide::le(const Wide& o) const {
-    for (int i = N - 1; i >= 0; --i)
-        if (v[i] != o.v[i]) return v[i] < o.v[i];
-    return true;
-}
-
-Wide Wide::shl(int n) const {
-    Wide r;
-    const int q = n / 64, b = n % 64;
-    for (int i = N - 1; i >= 0; --i) {
-        const int src = i - q;
-        if (src < 0) continue;
-        uint64_t x = v[src] << b;
-        if (b && src > 0) x |= v[src - 1] >> (64 - b);
-        r.v[i] = x;
-    }
-    if (bits() + n > 64 * N) throw std::overflow_error("oz2: shift overflow in plan arithmetic");
-    return r;
-}
-
-double Wide::log2_estimate() const {
-    // top three 32-bit words as a double, then scaled: the same estimate as the
-    // reference's Big::log2 (hpp:129-135), so tk's starting point matches it
-    int top = 2 * N - 1;
-    while (top > 0 && word32(top) == 0) --top;
-    double x = 0;
-    for (int i = top; i >= std::max(0, top - 2); --i) x = x * 4294967296.0 + word32(i);
-    return std::log2(x) + 32.0 * std::max(0, top - 2);
-}
-
-// ---- make_plan (hpp:159-200) -----------------------------------------------
+// ---- make_plan (what it computes: hpp:159-200) ------------------------------
 namespace {
-int64_t inverse_mod(int64_t a, int64_t m) {
-    int64_t r0 = m, r1 = ((a % m) + m) % m, t0 = 0, t1 = 1;
-    while (r1) {
-        const int64_t q = r0 / r1;
-        const int64_t r2 = r0 - q * r1, t2 = t0 - q * t1;
-        r0 = r1; r1 = r2; t0 = t1; t1 = t2;
+// inverse of a modulo p (0 < a < p, gcd 1): extended Euclid on small integers
+uint32_t inverse_mod_small(uint32_t a, uint32_t p) {
+    int64_t g0 = p, g1 = a, x0 = 0, x1 = 1;
+    while (g1 != 0) {
+        const int64_t q = g0 / g1;
+        int64_t t = g0 - q * g1;
+        g0 = g1;
+        g1 = t;
+        t = x0 - q * x1;
+        x0 = x1;
+        x1 = t;
     }
-    if (r0 != 1) throw std::runtime_error("oz2: moduli are not pairwise coprime");
-    return ((t0 % m) + m) % m;
+    if (g0 != 1) throw std::runtime_error("oz2: moduli are not pairwise coprime");
+    return static_cast<uint32_t>((x0 % p + p) % p);
 }
 }  // namespace
 
@@ -103,37 +73,44 @@ Plan make_plan(int s, int64_t k) {
     if (s < kMinModuli || s > kMaxModuli)
         throw std::invalid_argument("oz2: num_moduli must be in [2, " + std::to_string(kMaxModuli) + "]");
     if (k < 1) throw std::invalid_argument("oz2: k must be >= 1");
+    const uint16_t* set = moduli(s);
     Plan p;
     p.s = s;
     p.k = k;
-    const uint16_t* mods = moduli(s);
+    for (int t = 0; t < s; ++t) p.m[t] = set[t];
+
+    // CRT basis: W_t = y_t * prod_{u != t} m_u with y_t the inverse of that product
+    // modulo m_t.  The product's residue mod m_t comes from the small moduli directly,
+    // so no multi-word division is needed; M is the product of all of them.
     p.M = Wide::of(1);
+    for (int t = 0; t < s; ++t) p.M.mul(p.m[t]);
     for (int t = 0; t < s; ++t) {
-        if (mods[t] < 2 || mods[t] > 256) throw std::runtime_error("oz2: modulus out of int8 range");
-        p.M.mul(mods[t]);
+        Wide rest = Wide::of(1);
+        uint32_t rest_mod = 1;
+        for (int u = 0; u < s; ++u) {
+            if (u == t) continue;
+            rest.mul(p.m[u]);
+            rest_mod = rest_mod * (p.m[u] % p.m[t]) % p.m[t];
+        }
+        p.y[t] = inverse_mod_small(rest_mod, p.m[t]);
+        rest.mul(p.y[t]);
+        p.W[t] = rest;
     }
-    p.log2M = p.M.log2_estimate();
-    p.NL = std::max(2, static_cast<int>(std::floor(p.log2M / 32.0)) + 1);
+
+    // 32-bit words that hold M: floor(log2 M / 32) + 1 with log2 M taken from the bit length
+    // (M is no power of two), at least 2; the kernels' limb count per s must agree
+    p.NL = std::max(2, (p.M.bits() - 1) / 32 + 1);
     if (p.NL != limbs_for(s)) throw std::logic_error("oz2: limb count mismatch with the moduli table");
-    int tk = static_cast<int>(std::floor((p.log2M - 2.0 - std::log2(static_cast<double>(k))) / 2.0));
-    tk = std::min(tk, kMaxScaleBits);
-    for (;; --tk) {
-        if (tk < 1)
-            throw std::invalid_argument("oz2: num_moduli=" + std::to_string(s) + " is too small for k=" +
-                                        std::to_string(k));
-        if (Wide::of(static_cast<uint64_t>(k)).shl(2 * tk + 2).le(p.M)) break;
-    }
-    p.tk = tk;
-    for (int t = 0; t < s; ++t) {
-        const uint32_t m = mods[t];
-        Wide Mt = p.M;
-        if (Mt.divmod(m) != 0) throw std::runtime_error("oz2: bad modulus table");
-        const int64_t y = inverse_mod(Mt.mod(m), m);
-        p.W[t] = Mt;
-        p.W[t].mul(static_cast<uint32_t>(y));
-        p.m[t] = m;
-        p.y[t] = static_cast<uint32_t>(y);
-    }
+
+    // tk: the largest t <= kMaxScaleBits with k * 2^(2t+2) <= M.  Since the left side is an
+    // integer, k * 2^(2t+2) <= M  <=>  2^(2t+2) <= floor(M / k)  <=>  2t + 2 <= bits(floor(M / k)) - 1,
+    // so tk = (bits(floor(M / k)) - 3) / 2 rounded down, exactly, from one division.
+    Wide q = p.M;
+    q.divmod(static_cast<uint64_t>(k));
+    if (q.bits() < 5)  // floor(M / k) < 16: not even t = 1 fits
+        throw std::invalid_argument("oz2: num_moduli=" + std::to_string(s) + " is too small for k=" +
+                                    std::to_string(k));
+    p.tk = std::min((q.bits() - 3) / 2, kMaxScaleBits);
     return p;
 }
 

@@ -41,18 +41,15 @@ constexpr int limbs_for(int s) { return s <= 8 ? 2 : s <= 12 ? 3 : s <= 16 ? 4 :
 
 const uint16_t* moduli(int s);  // s entries, or nullptr
 
-// Unsigned fixed-width integer, 5 x 64-bit little-endian limbs (320 bits).
+// Unsigned fixed-width integer, 5 x 64-bit little-endian limbs (320 bits): the
+// products of up to 24 moduli below 2^8 (M < 2^192) and their multiples by y_t < 2^8.
 struct Wide {
     static constexpr int N = 5;
     uint64_t v[N] = {};
     static Wide of(uint64_t x) { Wide w; w.v[0] = x; return w; }
-    void mul(uint32_t f);                 // throws std::overflow_error
-    uint32_t divmod(uint32_t d);          // *this /= d, returns remainder
-    uint32_t mod(uint32_t d) const;
-    int bits() const;                     // bit length
-    bool le(const Wide& o) const;
-    Wide shl(int n) const;                // throws std::overflow_error
-    double log2_estimate() const;         // top 96 bits as double, same value as the reference's Big::log2
+    void mul(uint32_t f);                 // *this *= f; throws std::overflow_error
+    uint64_t divmod(uint64_t d);          // *this /= d (d > 0), returns the remainder
+    int bits() const;                     // bit length (0 for zero)
     uint32_t word32(int i) const { return static_cast<uint32_t>(v[i / 2] >> (32 * (i % 2))); }
 };
 
@@ -61,7 +58,6 @@ struct Plan {
     int64_t k = 0;
     int tk = 0;
     int NL = 0;
-    double log2M = 0;
     uint32_t m[kMaxModuli] = {};
     uint32_t y[kMaxModuli] = {};
     Wide W[kMaxModuli];

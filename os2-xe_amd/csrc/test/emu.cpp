@@ -128,4 +128,24 @@ void emu_crt(const uint8_t* res, const int32_t* shift, int64_t n, int s, double*
 }
 
 int emu_exponent_from_maxbits(uint64_t b, int tk) { return math::exponent_from_maxbits(b, tk); }
+
+// make_plan(s, k): tk, NL, y[t], W[t] and M as 5 little-endian 64-bit words each; returns 0, or 1 when
+// make_plan throws std::invalid_argument (s out of range, k < 1, s too small for k), 2 for other errors
+int emu_plan(int s, int64_t k, int* tk, int* nl, uint32_t* y, uint64_t* W, uint64_t* M) {
+    try {
+        const Plan p = make_plan(s, k);
+        *tk = p.tk;
+        *nl = p.NL;
+        for (int t = 0; t < s; ++t) {
+            y[t] = p.y[t];
+            for (int i = 0; i < Wide::N; ++i) W[t * Wide::N + i] = p.W[t].v[i];
+        }
+        for (int i = 0; i < Wide::N; ++i) M[i] = p.M.v[i];
+        return 0;
+    } catch (const std::invalid_argument&) {
+        return 1;
+    } catch (...) {
+        return 2;
+    }
+}
 }

@@ -24,6 +24,7 @@ _emu.emu_split_cplx.argtypes = [vp, vp, vp, i64, i32, i32, i32, vp]
 _emu.emu_acc_residue.argtypes = [vp, i64, i32, vp]
 _emu.emu_crt.argtypes = [vp, vp, i64, i32, vp]
 _emu.emu_exponent_from_maxbits.argtypes = [ctypes.c_uint64, i32]
+_emu.emu_plan.argtypes = [i32, i64, vp, vp, vp, vp, vp]
 
 
 def P(a):
@@ -171,3 +172,57 @@ def test_exponent_rule():
     for b, tk in ((0, 30), (0x7FF0000000000000, 30), (0x7FF8000000000000, 5), (1, 55), (0x3FF0000000000000, 55),
                   (0x000FFFFFFFFFFFFF, 55), (0x7FEFFFFFFFFFFFFF, 100)):
         assert _emu.emu_exponent_from_maxbits(b, tk) == O.exponent_from_maxbits(b, tk)
+
+
+def emu_plan(s, k):
+    tk, nl = ctypes.c_int(0), ctypes.c_int(0)
+    y = np.zeros(24, dtype=np.uint32)
+    W = np.zeros(24 * 5, dtype=np.uint64)
+    M = np.zeros(5, dtype=np.uint64)
+    rc = _emu.emu_plan(s, k, ctypes.byref(tk), ctypes.byref(nl), P(y), P(W), P(M))
+    if rc:
+        return rc, None
+
+    def big(words):
+        return sum(int(w) << (64 * i) for i, w in enumerate(words))
+    return 0, {"tk": tk.value, "NL": nl.value, "y": [int(v) for v in y[:s]],
+               "W": [big(W[5 * t:5 * t + 5]) for t in range(s)], "M": big(M)}
+
+
+def test_plan_matches_golden():
+    # the library's make_plan (exact integer form) against the fixture of exact Python integers
+    for g in load("plan.json")["plans"]:
+        rc, p = emu_plan(g["s"], g["k"])
+        if g["tk"] is None:
+            assert rc == 1, (g["s"], g["k"])
+            continue
+        assert rc == 0
+        assert p["tk"] == g["tk"] and p["NL"] == g["NL"] and p["y"] == g["y"], (g["s"], g["k"])
+        assert p["M"] == int(g["M"], 16) and p["W"] == [int(w, 16) for w in g["W"]]
+
+
+def test_plan_tk_at_every_boundary():
+    # tk is the largest t <= 100 with k 2^(2t+2) <= M: at k_t = floor(M / 2^(2t+2)) it is t (or the cap),
+    # one past it t - 1.  The oracle restates the reference's rule (a double-precision estimate of
+    # (log2 M - 2 - log2 k) / 2, then decrements): it agrees for every k below 2^50; from about 2^50.4 on
+    # (k that no GEMM in device memory can have) its estimate can start one below the largest t
+    for s in range(2, 25):
+        M = math.prod(O.moduli(s))
+        for t in range(1, 101):
+            kt = M >> (2 * t + 2)
+            if kt < 1:
+                break
+            for k in (kt - 1, kt, kt + 1):
+                if not 1 <= k < 2 ** 62:
+                    continue
+                want = max((u for u in range(1, 101) if (k << (2 * u + 2)) <= M), default=None)
+                rc, p = emu_plan(s, k)
+                if want is None:
+                    assert rc == 1 and O.precision_bits(k, s) < 0
+                else:
+                    assert rc == 0 and p["tk"] == want, (s, k)
+                    ref = O.precision_bits(k, s)
+                    if k < 2 ** 50:
+                        assert ref == want, (s, k)
+                    else:
+                        assert ref in (want, want - 1), (s, k)
