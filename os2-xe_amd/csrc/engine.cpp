@@ -12,37 +12,11 @@ namespace oz2 {
 
 namespace {
 inline int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
-inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string("oz2: ") + what + ": " + hipGetErrorString(e));
 }
-
-// MI355X cost-model constants for the tiling planner (measured on one GPU:
-// INT8 MFMA 32x32x32 4.9 Pop/s, streaming read 5.6 TB/s)
-constexpr double kInt8Rate = 4.0e15;
-constexpr double kHbmRate = 5.0e12;
-constexpr double kLaunch = 6e-6;
-
-// Plane strides that are multiples of 64 KiB put the same offset of every plane
-// on the same HBM channels; the GEMM (one plane per XCD range) and the CRT (all
-// planes of one entry) read those offsets at the same time.  Tuning::pad_lda
-// staggers such strides by 4 KiB, the counterpart of the reference's ld pad
-// for cache-set conflicts (ozaki2_gpu.hpp:669-672).
-constexpr int64_t kPlanePadPeriod = 64 << 10;
-constexpr int64_t kPlanePad = 4 << 10;
-inline int64_t plane_stride(int64_t bytes, bool pad) {
-    return pad && bytes % kPlanePadPeriod == 0 ? bytes + kPlanePad : bytes;
-}
 }  // namespace
-
-struct Engine::Layout {
-    k::Tile tile = k::Tile::T256;
-    int64_t TM = 256, TN = 256, kp = 0, mpad = 0, npad = 0, kch = 1;
-    int g = 1, planes = 1;
-    int64_t planeA = 0, planeB = 0, planeR = 0;
-    size_t oD = 0, oE = 0, oPA = 0, oPB = 0, oA = 0, oB = 0, oR = 0, total = 0;
-};
 
 Engine::Engine(hipStream_t stream) : stream_(stream) {
     std::vector<TabSet> host(kMaxModuli + 1);
@@ -176,102 +150,41 @@ size_t Engine::budget() const {
     return lim;
 }
 
-k::Tile Engine::pick_tile(int64_t mb, int64_t nb, int64_t kp, int planes) const {
-    k::Tile forced;
-    if (k::tile_from_code(tune_.gemm_tile, forced)) return forced;
-    // never pad a small tile up to a bigger macro tile
-    const int64_t lim = std::min(mb, nb);
-    if (lim <= 64) return k::Tile::T64;
-    auto blocks = [&](int64_t tm, int64_t tn) { return ((mb + tm - 1) / tm) * ((nb + tn - 1) / tn) * planes; };
-    // k loops of at most 1024: 256 x 128 tiles, two workgroups per CU (measured
-    // ahead of both square tiles there, behind 256 x 256 for long k)
-    if (kp <= 1024 && mb > 128 && nb > 64 && blocks(256, 128) >= 512) return k::Tile::T256x128;
-    // largest square tile that still gives every CU two workgroups
-    if (lim > 128 && blocks(256, 256) >= 512) return k::Tile::T256;
-    if (blocks(128, 128) >= 512) return k::Tile::T128;
-    return blocks(128, 128) >= 256 ? k::Tile::T128 : k::Tile::T64;
+k::Tile Engine::pick_tile(int64_t mb, int64_t nb, int64_t klen, int planes) const {
+    return oz2::pick_tile(mb, nb, klen, planes, tune_.gemm_tile);
 }
 
-Engine::Layout Engine::layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch,
-                              int64_t mb, int64_t nb) const {
-    Layout L;
-    L.g = cplx ? 3 : 1;
-    L.planes = L.g * s;
-    L.kch = kch;
-    L.kp = round_up(k, kKStep);
-    L.tile = pick_tile(mb, nb, k_chunk_len(L.kp), L.planes);
-    L.TM = k::tile_m(L.tile);
-    L.TN = k::tile_n(L.tile);
-    L.mpad = round_up(mb, L.TM);
-    L.npad = round_up(nb, L.TN);
-    L.planeA = plane_stride(L.mpad * L.kp, tune_.pad_lda);
-    L.planeB = plane_stride(L.npad * L.kp, tune_.pad_lda);
-    L.planeR = plane_stride(L.npad * L.mpad, tune_.pad_lda);
-    size_t off = 0;
-    auto carve = [&](size_t bytes) {
-        const size_t o = off;
-        off = align_up(off + bytes, 256);
-        return o;
-    };
-    L.oD = carve(static_cast<size_t>(m) * 4);
-    L.oE = carve(static_cast<size_t>(n) * 4);
-    L.oPA = carve(k::scale_partial_elems(m, k, ta != Op::N) * 8);
-    L.oPB = carve(k::scale_partial_elems(n, k, tb == Op::N) * 8);
-    L.oA = carve(static_cast<size_t>(L.planes) * L.planeA);
-    L.oB = carve(static_cast<size_t>(L.planes) * L.planeB);
-    L.oR = carve(static_cast<size_t>(kch) * L.planes * L.planeR);
-    L.total = off + 4096;
-    return L;
+CallShape Engine::call_shape(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx) const {
+    CallShape c;
+    c.m = m;
+    c.n = n;
+    c.k = k;
+    c.s = s;
+    c.cplx = cplx;
+    c.partial_a = k::scale_partial_elems(m, k, ta != Op::N) * 8;
+    c.partial_b = k::scale_partial_elems(n, k, tb == Op::N) * 8;
+    c.cus = cus_;
+    c.pad_planes = tune_.pad_lda;
+    c.pow2 = tune_.pow2_tiles;
+    c.per_plane = !tune_.batched_gemm;
+    c.forced_tile = tune_.gemm_tile;
+    return c;
+}
+
+TileInfo Engine::plan_for_call(const CallShape& c, size_t limit) {
+    if (!have_memo_ || memo_limit_ != limit || !same_call(memo_shape_, c)) {
+        memo_ti_ = plan_tiling(c, limit);
+        memo_shape_ = c;
+        memo_limit_ = limit;
+        have_memo_ = true;
+    }
+    return memo_ti_;
 }
 
 size_t Engine::workspace_for(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx) {
     // a call with an empty C, k = 0 or alpha = 0 only scales C and needs no workspace
     if (m == 0 || n == 0 || k == 0) return 0;
-    const int64_t kch = k_chunks(round_up(k, kKStep));
-    return choose_tiling(ta, tb, m, n, k, s, cplx, kch, budget()).workspace_bytes;
-}
-
-TileInfo Engine::choose_tiling(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch,
-                               size_t limit) const {
-    const double g = cplx ? 3.0 : 1.0;
-    auto candidates = [&](int64_t x) {
-        std::vector<int64_t> c{x};
-        for (int64_t p = 2; p <= 256; ++p) {
-            const int64_t b = round_up((x + p - 1) / p, 64);
-            if (b < x) c.push_back(b);
-        }
-        if (tune_.pow2_tiles)
-            for (int64_t b = 1024; b < x; b *= 2) c.push_back(b);
-        std::sort(c.begin(), c.end(), std::greater<>());
-        c.erase(std::unique(c.begin(), c.end()), c.end());
-        return c;
-    };
-    const auto nbs = candidates(n);
-    double best = std::numeric_limits<double>::infinity();
-    TileInfo bt;
-    for (int64_t mb : candidates(m)) {
-        const int64_t pm = (m + mb - 1) / mb;
-        for (int64_t nb : nbs) {
-            const int64_t pn = (n + nb - 1) / nb;
-            const Layout L = layout(ta, tb, m, n, k, s, cplx, kch, mb, nb);
-            if (L.total > limit) continue;
-            // padded GEMM work + HBM streams of split and CRT + launches (MI355X constants)
-            const double gemm = 2.0 * L.mpad * L.npad * L.kp * s * g * pm * pn / kInt8Rate;
-            const double splitA = g * (8.0 + s) * m * k * (pm > 1 ? pn : 1);
-            const double splitB = g * (8.0 + s) * k * n;
-            const double crt = (g * s * kch + 16.0 * (cplx ? 2 : 1)) * m * n;
-            const double t = gemm + (splitA + splitB + crt) / kHbmRate + pm * pn * (3 + kch) * kLaunch;
-            if (t < best) {
-                best = t;
-                bt = TileInfo{mb, nb, pm, pn, kch, L.total};
-            }
-            break;  // largest nb that fits for this mb
-        }
-    }
-    if (!std::isfinite(best))
-        throw std::runtime_error("oz2: not enough device memory for workspace (limit " + std::to_string(limit >> 20) +
-                                 " MiB)");
-    return bt;
+    return plan_for_call(call_shape(ta, tb, m, n, k, s, cplx), budget()).workspace_bytes;
 }
 
 void Engine::gemm(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
@@ -304,14 +217,14 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
     const bool wide = tk > kNarrowScaleBits;
     const TabSet* tab = dtabs_ + s;
 
-    const int64_t kp = round_up(k, kKStep);
-    const int64_t kch = k_chunks(kp);
-    const int64_t klen = k_chunk_len(kp);
-    tiling_ = choose_tiling(ta, tb, m, n, k, s, cplx, kch, budget());
-    const int64_t mb = tiling_.mb, nb = tiling_.nb;
-    const Layout L = layout(ta, tb, m, n, k, s, cplx, kch, mb, nb);
-    tile_ = L.tile;
+    const CallShape cs = call_shape(ta, tb, m, n, k, s, cplx);
+    const TileInfo ti = plan_for_call(cs, budget());
+    const int64_t mb = ti.mb, nb = ti.nb;
+    const Layout L = layout(cs, mb, nb);
+    const int64_t kp = L.kp, kch = L.kch, klen = L.klen;
     ensure_workspace(L.total);
+    tiling_ = ti;
+    tile_ = L.tile;
 
     int32_t* D = reinterpret_cast<int32_t*>(ws_ + L.oD);
     int32_t* E = reinterpret_cast<int32_t*>(ws_ + L.oE);

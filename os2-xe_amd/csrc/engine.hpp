@@ -15,18 +15,12 @@
 
 #include "kernels.hpp"
 #include "plan.hpp"
+#include "planner.hpp"
 
 namespace oz2 {
 
 struct StageTimes {
     double scale_ms = 0, split_ms = 0, gemm_ms = 0, crt_ms = 0;
-};
-
-struct TileInfo {
-    int64_t mb = 0, nb = 0;
-    int64_t pm = 0, pn = 0;
-    int64_t k_chunks = 0;
-    size_t workspace_bytes = 0;
 };
 
 struct Tuning {
@@ -76,7 +70,7 @@ public:
     // shared with batch plans
     const TabSet* device_tables() const { return dtabs_; }
     const Plan& plan_for(int s, int64_t k);
-    k::Tile pick_tile(int64_t mb, int64_t nb, int64_t kp, int planes) const;
+    k::Tile pick_tile(int64_t mb, int64_t nb, int64_t klen, int planes) const;
     size_t budget() const;
     int compute_units() const { return cus_; }
 
@@ -85,11 +79,9 @@ private:
     void run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, double ai, const double* A, int64_t lda,
              const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s, bool cplx);
     void record(Stage st);
-    struct Layout;
-    Layout layout(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch, int64_t mb,
-                  int64_t nb) const;
-    TileInfo choose_tiling(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx, int64_t kch,
-                           size_t limit) const;
+    CallShape call_shape(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx) const;
+    // plan_tiling, remembered for the last shape and limit
+    TileInfo plan_for_call(const CallShape& c, size_t limit);
     void ensure_workspace(size_t bytes);
 
     hipStream_t stream_;
@@ -105,6 +97,10 @@ private:
     size_t ws_limit_ = 0;
     size_t resident_hint_ = 0;
     TileInfo tiling_{};
+    bool have_memo_ = false;
+    CallShape memo_shape_{};
+    size_t memo_limit_ = 0;
+    TileInfo memo_ti_{};
     k::Tile tile_ = k::Tile::T256;
     std::vector<hipEvent_t> events_;
     std::vector<std::pair<Stage, size_t>> stages_;

@@ -319,22 +319,6 @@ __global__ void reset_counters(uint32_t* c, int n) {
 
 }  // namespace
 
-int tile_m(Tile t) { return t == Tile::T256 || t == Tile::T256x128 ? 256 : t == Tile::T128 ? 128 : 64; }
-int tile_n(Tile t) { return t == Tile::T256 ? 256 : t == Tile::T64 ? 64 : 128; }
-int tile_code(Tile t) { return t == Tile::T256x128 ? 256128 : tile_m(t); }
-bool tile_from_code(int code, Tile& t) {
-    switch (code) {
-        case 256: t = Tile::T256; return true;
-        case 128: t = Tile::T128; return true;
-        case 64: t = Tile::T64; return true;
-        case 256128: t = Tile::T256x128; return true;
-        default: return false;
-    }
-}
-
-// workgroups per CU (LDS: 128 / 64 / 32 / 72 KB per workgroup)
-static int tile_blocks_per_cu(Tile t) { return t == Tile::T256 ? 1 : t == Tile::T64 ? 4 : 2; }
-
 void prep_gemm(std::vector<GroupedLaunch>& out, const std::vector<GemmJob>& jobs) {
     GroupedLaunch by[4];
     for (const GemmJob& j : jobs) {

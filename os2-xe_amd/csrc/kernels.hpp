@@ -14,6 +14,7 @@
 
 #include "grouped.hpp"
 #include "plan.hpp"
+#include "planner.hpp"
 
 namespace oz2 {
 namespace k {
@@ -78,15 +79,6 @@ struct GemmArgs {
     int s;
     const TabSet* tab;    // device tables of this s
 };
-// Macro tiles of the INT8 GEMM (rows of op(A) x columns of op(B) per workgroup of
-// 4 waves).  T256: one workgroup per CU, 4 LDS stages (long k).  T256x128: two
-// workgroups per CU, 3 stages, so one workgroup's residue epilogue runs while the
-// other's MFMAs do (short k).  T128, T64: small problems.
-enum class Tile { T256 = 0, T128 = 1, T64 = 2, T256x128 = 3 };
-int tile_m(Tile t);              // rows of op(A) per tile
-int tile_n(Tile t);              // columns of op(B) per tile
-int tile_code(Tile t);           // API code: 256, 128, 64 (square) or 256128
-bool tile_from_code(int code, Tile& t);
 void igemm_residue(hipStream_t st, Tile tile, const GemmArgs& a);
 
 // ---- CRT + alpha/beta epilogue -------------------------------------------------

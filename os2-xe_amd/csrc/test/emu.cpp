@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "../math.hpp"
+#include "../planner.hpp"
 
 using namespace oz2;
 
@@ -148,4 +149,40 @@ int emu_plan(int s, int64_t k, int* tk, int* nl, uint32_t* y, uint64_t* W, uint6
         return 2;
     }
 }
+
+// ---- panel planner (csrc/planner.cpp) ----
+// shape: m, n, k, s, cplx, partial_a, partial_b, cus, pad, pow2, per_plane, forced_tile
+static CallShape emu_shape(const int64_t* sh) {
+    CallShape c;
+    c.m = sh[0];
+    c.n = sh[1];
+    c.k = sh[2];
+    c.s = static_cast<int>(sh[3]);
+    c.cplx = sh[4] != 0;
+    c.partial_a = static_cast<size_t>(sh[5]);
+    c.partial_b = static_cast<size_t>(sh[6]);
+    c.cus = static_cast<int>(sh[7]);
+    c.pad_planes = sh[8] != 0;
+    c.pow2 = sh[9] != 0;
+    c.per_plane = sh[10] != 0;
+    c.forced_tile = static_cast<int>(sh[11]);
+    return c;
+}
+// out: mb, nb, pm, pn, k_chunks, workspace_bytes; returns 0, or 1 when nothing fits
+int emu_plan_tiling(const int64_t* sh, uint64_t limit, int64_t* out) {
+    try {
+        const TileInfo t = plan_tiling(emu_shape(sh), limit);
+        out[0] = t.mb;
+        out[1] = t.nb;
+        out[2] = t.pm;
+        out[3] = t.pn;
+        out[4] = t.k_chunks;
+        out[5] = static_cast<int64_t>(t.workspace_bytes);
+        return 0;
+    } catch (const std::runtime_error&) {
+        return 1;
+    }
+}
+uint64_t emu_layout_total(const int64_t* sh, int64_t mb, int64_t nb) { return layout(emu_shape(sh), mb, nb).total; }
+double emu_panel_cost(const int64_t* sh, int64_t mb, int64_t nb) { return panel_cost(emu_shape(sh), mb, nb); }
 }
