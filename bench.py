@@ -27,7 +27,6 @@ for _p in (ROOT, os.path.join(ROOT, "os2-xe_amd")):
 
 INT8_PEAK_TOPS = 5000.0  # MI355X dense INT8 MFMA, 5.0 POP/s
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E, 8 TB/s
-HBM_PEAK_GBS = 8000.0
 
 
 # HBM bytes per INT8 GEMM launch measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE passes,
@@ -99,27 +98,8 @@ def accuracy(oz, torch, dA, dB, m, n, k, s):
 
 
 def make_batch(torch, count, seed):
-    """uneven batch: m, n, k log-uniform in [64, 4096]; even entries U[-1,1), odd entries
-    (1 + U[0,1)) * (+-1) * 2^e with e uniform in [-12, 12]; alpha = 1, beta = 0"""
-    import math
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    probs = []
-    for i in range(count):
-        m, n, k = (int(round(math.exp(rng.uniform(math.log(64), math.log(4096))))) for _ in range(3))
-        wide = i % 2 == 1
-
-        def gen(cnt):
-            if not wide:
-                return torch.rand(cnt, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
-            e = torch.randint(-12, 13, (cnt,), device="cuda", generator=g).double()
-            sg = torch.randint(0, 2, (cnt,), device="cuda", generator=g).double() * 2 - 1
-            return (1 + torch.rand(cnt, dtype=torch.float64, device="cuda", generator=g)) * sg * torch.exp2(e)
-        probs.append(dict(ta="N", tb="N", m=m, n=n, k=k, alpha=1.0, beta=0.0, A=gen(m * k), lda=m, B=gen(k * n),
-                          ldb=k, C=torch.empty(m * n, dtype=torch.float64, device="cuda"), ldc=m, wide=wide))
-    return probs
+    from oz2.workloads import uneven_batch
+    return uneven_batch(torch, count, seed)
 
 
 def batch_bench(oz, torch, count, reps=5, bits=53):
@@ -327,26 +307,66 @@ def selector_line(args, torch, which):
             "config": {"workload": f"DGEMM m={m} n={n} k={k} N/N alpha=1 beta=0, 53-bit target"}, "runs": res}
 
 
+def _host_cpu():
+    """CPU model name (lscpu) and the logical CPUs of the host"""
+    import subprocess
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        model = next((ln.split(":", 1)[1].strip() for ln in out.splitlines() if ln.startswith("Model name")), "")
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return model, os.cpu_count()
+
+
 def cpu_baseline(m, n, k, s):
-    """Oracle (CPU restatement, OpenMP) on the first `rows` rows of op(A): same k, n, s, distribution."""
+    """The CPU-runnable configuration (DGEMM 1024^3 N/N, alpha=1, beta=0, U[-1,1), s=14) through the
+    oracle, a C/OpenMP restatement of the reference's algorithm (its SYCL CPU device cannot run here):
+    1 warm-up, then best and median wall time of 5 calls, checked against a double-double product with
+    the reference's rigorous bound.  Also one call on rows 0..3071 of the GPU line's DGEMM."""
     import numpy as np
     from oracle import oracle as O
     threads = len(os.sched_getaffinity(0))
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    model, ncpu = _host_cpu()
+    N, s1 = 1024, 14
     rng = np.random.default_rng(1)
-    rows = 3072  # about 10-15 s of OpenMP work on 16 host cores
+    A = rng.uniform(-1, 1, N * N)
+    B = rng.uniform(-1, 1, N * N)
+    C = np.zeros(N * N)
+    O.dgemm("N", "N", N, N, N, 1.0, A, N, B, N, 0.0, C, N, s1, threads)  # warm-up
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        O.dgemm("N", "N", N, N, N, 1.0, A, N, B, N, 0.0, C, N, s1, threads)
+        ts.append(time.perf_counter() - t0)
+    hi, lo = O.dd_gemm("N", "N", N, N, N, A, N, B, N, threads)
+    bound = O.bound_ratio("N", "N", N, N, N, A, N, B, N, 1.0, 0.0, np.zeros(N * N), C, N, hi, lo, s1)
+    acc = O.metrics(N, N, C, N, hi, lo)
+    ts.sort()
+    flops = 2.0 * N ** 3
+    out = {"value": round(flops / ts[0] / 1e12, 5), "unit": "FP64-equiv TFLOP/s", "cores": threads, "kind": "port",
+           "label": "CPU restatement (reference SYCL CPU device unavailable)",
+           "sample": f"DGEMM {N}x{N}x{N} N/N alpha=1 beta=0 U[-1,1) seed 1, s={s1}: oracle residue route "
+                     f"(split, exact int8 GEMM, CRT) with OpenMP on {threads} threads; 1 warm-up, best of 5",
+           "best_s": round(ts[0], 4), "median_s": round(ts[2], 4),
+           "median_tflops": round(flops / ts[2] / 1e12, 5),
+           "nproc": ncpu, "cpu_model": model,
+           "accuracy_vs_double_double": {"bound_ratio": bound, **acc}}
+    # the GPU line's own workload, on its first 3072 rows (about 10-15 s on 16 host cores)
+    rows = 3072
+    rng = np.random.default_rng(1)
     A = rng.uniform(-1, 1, rows * k)
     B = rng.uniform(-1, 1, k * n)
     C = rng.uniform(-1, 1, rows * n)
-    O.dgemm("N", "N", 8, 64, 256, 1.5, A, rows, B, k, 0.5, C, rows, s, threads)  # load / warm
     t0 = time.perf_counter()
     O.dgemm("N", "N", rows, n, k, 1.5, A, rows, B, k, 0.5, C, rows, s, threads)
     dt = time.perf_counter() - t0
-    return {"value": 2.0 * rows * n * k / dt / 1e12, "unit": "FP64-equiv TFLOP/s", "cores": threads, "kind": "port",
-            "seconds": dt,
-            "sample": f"rows 0..{rows - 1} of the {m}x{n}x{k} s={s} DGEMM (alpha=1.5, beta=0.5): {rows}x{n}x{k}, "
-                      f"oracle residue route (split, exact int8 GEMM, CRT) with OpenMP"}
+    out["gpu_workload_sample"] = {"tflops": round(2.0 * rows * n * k / dt / 1e12, 5), "seconds": round(dt, 3),
+                                  "sample": f"rows 0..{rows - 1} of the {m}x{n}x{k} s={s} DGEMM (alpha=1.5, "
+                                            f"beta=0.5): {rows}x{n}x{k}, one call"}
+    return out
 
 
 def native_fp64(torch, m, n, k, dA, dB, cplx=False, op_a="N", lda=None, reps=5):

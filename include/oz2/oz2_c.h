@@ -90,7 +90,9 @@ int oz2_moduli(int num_moduli, uint16_t* out);
  * a grouped launch sequence (scale, split, one persistent INT8 GEMM per macro-tile shape, CRT: about
  * ten launches for any count); with OZ2_BATCH_GRAPH that sequence is captured in a hipGraph when the
  * plan is created and oz2_batch_run replays it.  Buffers are bound at creation: running the plan
- * again recomputes from their current contents.  Each problem's C is bit-identical to oz2_dgemm with
+ * again recomputes from their current contents.  The moduli chosen at creation (by the selector or
+ * moduli_for_bits) stay fixed: a run over new contents keeps them, even where the new data would need
+ * more moduli for the bit target; oz2_batch_reselect re-runs the selector on the current contents.  Each problem's C is bit-identical to oz2_dgemm with
  * the same num_moduli.  tiles: NULL (per-problem heuristic) or count macro-tile codes: 256, 128, 64
  * (square tiles of that many rows and columns), 256128 (256 rows of op(A) x 128 columns of op(B),
  * two workgroups per CU: faster for k up to about 1024) or 0 (heuristic).  oz2_batch_run enqueues on the handle's current stream; oz2_batch_destroy waits for
@@ -117,6 +119,11 @@ enum { OZ2_BATCH_GRAPH = 1, OZ2_BATCH_SELECT = 2, OZ2_BATCH_PERSISTENT = 4, OZ2_
 int oz2_batch_create(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, const int* tiles, int flags,
                      oz2_batch* out);
 int oz2_batch_run(oz2_batch b);
+/* For plans made with OZ2_BATCH_SELECT: runs the selector again on the buffers' current contents
+ * (synchronizing the handle's stream) and, if any problem's choice changed, rebuilds the plan with the
+ * same tiles and flags (new workspace; a new graph with OZ2_BATCH_GRAPH).  *changed (may be NULL):
+ * the number of problems whose num_moduli changed; 0 and nothing done without OZ2_BATCH_SELECT. */
+int oz2_batch_reselect(oz2_batch b, int* changed);
 /* Autotuner: measures whole runs of candidate plans for these problems (tile policies x GEMM
  * schedules) and returns the fastest: tiles_out (count ints, for oz2_batch_create), flags_out (the
  * OZ2_BATCH_PERSISTENT / OZ2_BATCH_PROBLEM_ORDER bits to pass on), best_ms.  Runs the batch about
@@ -174,6 +181,14 @@ int oz2_unit_split(void* stream, const double* X, int64_t ldx, int64_t rows, int
                    const int32_t* ex, int num_moduli, int tk, int8_t* planes, int64_t rows_pad, int64_t kp);
 int oz2_unit_crt(void* stream, const uint8_t* R, int64_t ldr, int64_t mb, int64_t nb, const int32_t* D,
                  const int32_t* E, int num_moduli, double alpha, double beta, double* C, int64_t ldc);
+/* The INT8 GEMM with its residue epilogue as one launch: the counterpart of the reference's oneMKL
+ * gemm_batch call (hpp:1029-1033), whose int32 products its tests pin only end to end.  Planes of op(A)
+ * (num_moduli x mpad x kp) and of op(B) (num_moduli x npad x kp), blocked as oz2_unit_split writes them
+ * -> R[t][j][ldr] = (sum_p op(A)_t[i][p] op(B)_t[j][p]) mod m_t as a byte in [0, m_t), for every
+ * i < mpad, j < npad.  tile: 256, 128, 64 or 256128 (rows x columns of the macro tile; mpad and npad
+ * multiples of it); kp a multiple of 128 up to 65408 (one k-chunk); ldr >= mpad, a multiple of 16. */
+int oz2_unit_gemm(void* stream, const int8_t* A, const int8_t* B, int64_t mpad, int64_t npad, int64_t kp,
+                  int num_moduli, int tile, uint8_t* R, int64_t ldr);
 
 const char* oz2_last_error(void);
 const char* oz2_version(void);

@@ -15,58 +15,24 @@
 #include "batch.hpp"
 #include "selector.hpp"
 #include "engine.hpp"
-
-struct oz2_engine {
-    oz2::Engine eng;
-    oz2::Selector sel;
-    explicit oz2_engine(hipStream_t s) : eng(s) {}
-};
+#include "capi.hpp"
 
 struct oz2_batch_s {
     oz2_engine* h = nullptr;
     std::unique_ptr<oz2::BatchPlan> plan;
     int count = 0;
+    // what the plan was made from, for oz2_batch_reselect
+    std::vector<oz2_gemm_desc_t> descs;
+    std::vector<int> tiles;
+    int bits = 0, flags = 0;
 };
 
 namespace {
-thread_local std::string g_err;
-
-template <class F>
-int guard(F&& f) {
-    try {
-        f();
-        return OZ2_OK;
-    } catch (const std::invalid_argument& e) {
-        g_err = e.what();
-        return OZ2_INVALID_ARGUMENT;
-    } catch (const std::overflow_error& e) {
-        g_err = e.what();
-        return OZ2_OVERFLOW_ERROR;
-    } catch (const std::logic_error& e) {
-        g_err = e.what();
-        return OZ2_LOGIC_ERROR;
-    } catch (const std::bad_alloc&) {
-        g_err = "oz2: host allocation failed";
-        return OZ2_RUNTIME_ERROR;
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return OZ2_RUNTIME_ERROR;
-    } catch (...) {
-        g_err = "oz2: unknown error";
-        return OZ2_RUNTIME_ERROR;
-    }
-}
-
-oz2::Op to_op(int t) {
-    if (t == OZ2_TRANS_N) return oz2::Op::N;
-    if (t == OZ2_TRANS_T) return oz2::Op::T;
-    if (t == OZ2_TRANS_C) return oz2::Op::C;
-    throw std::invalid_argument("oz2: bad transpose code");
-}
-
-void need(oz2_handle h) {
-    if (!h) throw std::invalid_argument("oz2: null handle");
-}
+using oz2::capi::guard;
+using oz2::capi::need;
+using oz2::capi::signal;
+using oz2::capi::to_op;
+using oz2::capi::wait_deps;
 
 oz2::SelProblem sel_problem(const oz2::GemmDesc& g, int bits) {
     return oz2::SelProblem{g.ta == oz2::Op::N ? 0 : 1, g.tb == oz2::Op::N ? 0 : 1, g.m, g.n, g.k, g.A, g.lda, g.B,
@@ -99,13 +65,6 @@ void select_moduli(oz2_handle h, const std::vector<oz2::GemmDesc>& d, int bits, 
     for (size_t j = 0; j < idx.size(); ++j) s[idx[j]] = c[j];
 }
 
-void wait_deps(hipStream_t st, void* const* deps, int ndeps) {
-    for (int i = 0; i < ndeps; ++i)
-        if (deps && deps[i])
-            if (hipStreamWaitEvent(st, static_cast<hipEvent_t>(deps[i]), 0) != hipSuccess)
-                throw std::runtime_error("oz2: hipStreamWaitEvent failed");
-}
-
 // device tables for the kernel-level entry points: one copy per device, kept until exit
 const oz2::TabSet* unit_tables() {
     static std::mutex mu;
@@ -124,11 +83,6 @@ const oz2::TabSet* unit_tables() {
         throw std::runtime_error("oz2: table upload failed");
     per_dev[dev] = static_cast<oz2::TabSet*>(p);
     return per_dev[dev];
-}
-
-void signal(hipStream_t st, void* done) {
-    if (done && hipEventRecord(static_cast<hipEvent_t>(done), st) != hipSuccess)
-        throw std::runtime_error("oz2: hipEventRecord failed");
 }
 
 // descriptors and moduli of a batch as oz2_batch_create chooses them
@@ -156,6 +110,15 @@ oz2::BatchOptions batch_options(int flags) {
     return o;
 }
 }  // namespace
+
+namespace oz2 {
+namespace capi {
+std::string& last_error() {
+    thread_local std::string err;
+    return err;
+}
+}  // namespace capi
+}  // namespace oz2
 
 extern "C" {
 
@@ -332,9 +295,34 @@ int oz2_batch_create(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int 
         auto b = std::make_unique<oz2_batch_s>();
         b->h = h;
         b->count = count;
+        b->descs.assign(descs, descs + count);
+        b->tiles = t;
+        b->bits = bits;
+        b->flags = flags;
         b->plan = std::make_unique<oz2::BatchPlan>(h->eng, std::move(d), s, t, (flags & OZ2_BATCH_GRAPH) != 0,
                                                    batch_options(flags));
         *out = b.release();
+    });
+}
+
+int oz2_batch_reselect(oz2_batch b, int* changed) {
+    return guard([&] {
+        if (!b) throw std::invalid_argument("oz2: null batch");
+        int nchanged = 0;
+        if ((b->flags & OZ2_BATCH_SELECT) != 0) {
+            std::vector<oz2::GemmDesc> d;
+            std::vector<int> s;
+            // the selector reads the buffers' current contents (it synchronizes the stream first)
+            batch_inputs(b->h, b->count, b->descs.data(), b->bits, b->flags, d, s);
+            const std::vector<int>& old = b->plan->moduli();
+            for (int i = 0; i < b->count; ++i) nchanged += s[i] != old[i];
+            if (nchanged) {
+                b->plan.reset();  // waits for the old plan's work and frees its workspace and graph
+                b->plan = std::make_unique<oz2::BatchPlan>(b->h->eng, std::move(d), s, b->tiles,
+                                                           (b->flags & OZ2_BATCH_GRAPH) != 0, batch_options(b->flags));
+            }
+        }
+        if (changed) *changed = nchanged;
     });
 }
 
@@ -528,7 +516,40 @@ int oz2_unit_crt(void* stream, const uint8_t* R, int64_t ldr, int64_t mb, int64_
     });
 }
 
-const char* oz2_last_error(void) { return g_err.c_str(); }
+int oz2_unit_gemm(void* stream, const int8_t* A, const int8_t* B, int64_t mpad, int64_t npad, int64_t kp,
+                  int num_moduli, int tile, uint8_t* R, int64_t ldr) {
+    return guard([&] {
+        if (num_moduli < oz2::kMinModuli || num_moduli > oz2::kMaxModuli)
+            throw std::invalid_argument("oz2: num_moduli must be in [2, 24]");
+        oz2::k::Tile t;
+        if (!oz2::k::tile_from_code(tile, t)) throw std::invalid_argument("oz2: tile must be 64, 128, 256 or 256128");
+        const int64_t TM = oz2::k::tile_m(t), TN = oz2::k::tile_n(t);
+        if (mpad <= 0 || npad <= 0 || mpad % TM || npad % TN || kp <= 0 || kp % oz2::kKStep || kp > oz2::kMaxKChunk ||
+            ldr < mpad || ldr % 16 || (mpad / TM) * (npad / TN) * num_moduli > (int64_t{1} << 31))
+            throw std::invalid_argument("oz2: bad unit_gemm shape");
+        oz2::k::GemmArgs a{};
+        a.A = A;
+        a.B = B;
+        a.R = R;
+        a.kp = kp;
+        a.plane_a = mpad * kp;
+        a.plane_b = npad * kp;
+        a.plane_r = ldr * npad;
+        a.ldr = ldr;
+        a.k0 = 0;
+        a.klen = kp;
+        a.tiles_m = static_cast<int>(mpad / TM);
+        a.tiles_n = static_cast<int>(npad / TN);
+        a.nplanes = num_moduli;
+        a.plane0 = 0;
+        a.s = num_moduli;
+        a.tab = unit_tables() + num_moduli;
+        oz2::k::igemm_residue(static_cast<hipStream_t>(stream), t, a);
+        if (hipGetLastError() != hipSuccess) throw std::runtime_error("oz2: gemm launch failed");
+    });
+}
+
+const char* oz2_last_error(void) { return oz2::capi::last_error().c_str(); }
 const char* oz2_version(void) { return "oz2-mi355x 0.1"; }
 
 }  // extern "C"

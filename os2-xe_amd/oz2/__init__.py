@@ -100,6 +100,7 @@ _lib.oz2_profile_begin.argtypes = [_vp]
 _lib.oz2_profile_end.argtypes = [_vp, _P(StageTimes), _P(_i64)]
 _lib.oz2_unit_split.argtypes = [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp, _int, _int, _vp, _i64, _i64]
 _lib.oz2_unit_crt.argtypes = [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _int, _dbl, _dbl, _vp, _i64]
+_lib.oz2_unit_gemm.argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, _int, _int, _vp, _i64]
 
 
 class GemmDesc(ctypes.Structure):
@@ -111,6 +112,7 @@ class GemmDesc(ctypes.Structure):
 
 _lib.oz2_batch_create.argtypes = [_vp, _int, _P(GemmDesc), _int, _P(_int), _int, _P(_vp)]
 _lib.oz2_batch_run.argtypes = [_vp]
+_lib.oz2_batch_reselect.argtypes = [_vp, _P(_int)]
 _lib.oz2_batch_moduli.argtypes = [_vp, _P(_int)]
 _lib.oz2_batch_tiles.argtypes = [_vp, _P(_int)]
 _lib.oz2_batch_info.argtypes = [_vp, _P(_sz), _P(_int), _P(_int)]
@@ -134,7 +136,7 @@ EXPORTS = [
     "oz2_precision_bits_complex", "oz2_moduli_for_bits", "oz2_set_workspace_limit", "oz2_set_resident_hint",
     "oz2_release_workspace", "oz2_reserve_workspace", "oz2_workspace_query", "oz2_stage_times", "oz2_last_tiling",
     "oz2_set_tuning", "oz2_get_tuning", "oz2_set_profiling", "oz2_profile_begin", "oz2_profile_end", "oz2_moduli",
-    "oz2_unit_split", "oz2_unit_crt", "oz2_batch_create", "oz2_batch_run", "oz2_batch_moduli", "oz2_batch_tiles",
+    "oz2_unit_split", "oz2_unit_crt", "oz2_unit_gemm", "oz2_batch_create", "oz2_batch_run", "oz2_batch_reselect", "oz2_batch_moduli", "oz2_batch_tiles",
     "oz2_batch_info", "oz2_batch_destroy", "oz2_batch_autotune", "oz2_selector_features", "oz2_selector_choose",
     "oz2_selector_set_weights", "oz2_selector_get_weights", "oz2_mlp_forward", "oz2_mlp_backward", "oz2_mlp_adam",
     "oz2_last_error", "oz2_version",
@@ -182,6 +184,11 @@ def unit_crt(R, ldr, mb, nb, D, E, num_moduli, alpha, beta, C, ldc, stream=None)
                              _ptr(C), ldc))
 
 
+def unit_gemm(A, B, mpad, npad, kp, num_moduli, tile, R, ldr, stream=None):
+    """Kernel-level INT8 GEMM with the residue epilogue (the reference's gemm_batch call)."""
+    _check(_lib.oz2_unit_gemm(stream, _ptr(A), _ptr(B), mpad, npad, kp, num_moduli, int(tile), _ptr(R), ldr))
+
+
 def _descs(problems):
     n = len(problems)
     arr = (GemmDesc * max(1, n))()
@@ -196,7 +203,8 @@ def _descs(problems):
 class Batch:
     """A plan for many DGEMMs (oz2_batch_create).  problems: dicts with keys ta, tb, m, n, k, alpha, A, lda,
     B, ldb, beta, C, ldc and optional num_moduli (<= 0 or absent: chosen by the plan for `bits`).
-    The plan keeps the tensors it was given alive; run() recomputes from their current contents."""
+    The plan keeps the tensors it was given alive; run() recomputes from their current contents with the
+    moduli chosen at creation; reselect() chooses again from the current contents."""
 
     def __init__(self, oz, problems, bits=53, tiles=None, graph=True, select=False, persistent=False,
                  largest_first=True):
@@ -215,6 +223,13 @@ class Batch:
 
     def run(self):
         _check(_lib.oz2_batch_run(self._b))
+
+    def reselect(self):
+        """re-run the selector on the buffers' current contents (oz2_batch_reselect); returns how many
+        problems' num_moduli changed"""
+        changed = _int(0)
+        _check(_lib.oz2_batch_reselect(self._b, ctypes.byref(changed)))
+        return changed.value
 
     def moduli(self):
         out = (_int * max(1, self.count))()

@@ -1,9 +1,9 @@
 """Batched DGEMMs (oz2_batch_*): one grouped launch sequence for many problems of
 different shapes, optionally replayed from a hipGraph.  Every problem's C must be
-bit-identical to oz2_dgemm with the same num_moduli (which the other GPU tests pin
-to the oracle), whatever the macro tiles, graph use or replay count; problems with
-k == 0 or alpha == 0 follow the C = beta*C rule (hpp:750-751) and beta == 0 never
-reads C (hpp:623)."""
+bit-identical to the oracle (the reference's algorithm restated, run on the same
+inputs with the same num_moduli) and to oz2_dgemm, whatever the macro tiles, graph
+use or replay count; problems with k == 0 or alpha == 0 follow the C = beta*C rule
+(hpp:750-751) and beta == 0 never reads C (hpp:623)."""
 import numpy as np
 import pytest
 import torch
@@ -11,6 +11,7 @@ import torch
 import oz2
 from conftest import make_mat
 from gpu_cases import bits, flat, log, to_dev
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -51,14 +52,24 @@ def build(problems, seed=7):
         if beta == 0.0 and m and n:
             C0[:m, :] = np.nan  # must not be read
         out.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=alpha, beta=beta, s=s, lda=lda, ldb=ldb, ldc=ldc,
-                        A=to_dev(flat(A)), B=to_dev(flat(B)), C0=flat(C0)))
+                        A=to_dev(flat(A)), B=to_dev(flat(B)), C0=flat(C0), Ah=flat(A), Bh=flat(B)))
     return out
+
+
+def oracle_c(p):
+    """the oracle's C for problem p (host copies of its inputs)"""
+    C = p["C0"].copy()
+    if p["m"] and p["n"]:
+        O.dgemm(p["ta"], p["tb"], p["m"], p["n"], p["k"], p["alpha"], p["Ah"], p["lda"], p["Bh"], p["ldb"], p["beta"], C,
+                p["ldc"], p["s"])
+    return C
 
 
 def reference(oz, p):
     C = to_dev(p["C0"].copy())
     oz.gemm(p["ta"], p["tb"], p["m"], p["n"], p["k"], p["alpha"], p["A"], p["lda"], p["B"], p["ldb"], p["beta"], C,
             p["ldc"], p["s"])
+    p["want"] = oracle_c(p)
     return C
 
 
@@ -67,11 +78,16 @@ def desc(p, C, s=None):
                 B=p["B"], ldb=p["ldb"], beta=p["beta"], C=C, ldc=p["ldc"], num_moduli=p["s"] if s is None else s)
 
 
-def check_same(ref, got, what):
+def check_same(ref, got, what, ps=None):
+    """every batch result against oz2_dgemm's (ref) and, given the problems, against the oracle"""
     for i, (r, g) in enumerate(zip(ref, got)):
         a, b = r.cpu().numpy(), g.cpu().numpy()
         nbad = int(np.count_nonzero(bits(a) != bits(b)))
         assert nbad == 0, f"{what}: problem {i} has {nbad} entries different from oz2_dgemm"
+        if ps is not None:
+            w = ps[i]["want"] if "want" in ps[i] else oracle_c(ps[i])
+            nbad = int(np.count_nonzero(bits(b) != bits(w)))
+            assert nbad == 0, f"{what}: problem {i} has {nbad} entries different from the oracle"
 
 
 @pytest.mark.parametrize("graph", [False, True])
@@ -86,7 +102,7 @@ def test_batch_matches_single_calls(oz, graph):
     assert info["graph"] == graph and info["launches"] < 25, info
     b.run()
     torch.cuda.synchronize()
-    check_same(ref, Cs, "first run")
+    check_same(ref, Cs, "first run", ps)
     b.close()
 
 
@@ -100,7 +116,7 @@ def test_forced_tiles_identical(oz):
         assert all(x in (0, t) for x in b.tiles())
         b.run()
         torch.cuda.synchronize()
-        check_same(ref, Cs, f"tile {t}")
+        check_same(ref, Cs, f"tile {t}", ps)
         b.close()
 
 
@@ -116,10 +132,12 @@ def test_graph_replay_reads_current_inputs(oz):
             for p in ps:
                 p["A"].mul_(-0.5)
                 p["B"].add_(0.25)
+                p["Ah"] = p["Ah"] * -0.5  # the same IEEE operations on the host copies
+                p["Bh"] = p["Bh"] + 0.25
         b.run()
         torch.cuda.synchronize()
         ref = [reference(oz, p) for p in ps]
-        check_same(ref, Cs, f"replay {rep}")
+        check_same(ref, Cs, f"replay {rep}", ps)
     b.close()
 
 
@@ -131,7 +149,7 @@ def test_k_chunks_in_batch(oz):
     b = oz2.Batch(oz, [desc(p, C) for p, C in zip(ps, Cs)], graph=True)
     b.run()
     torch.cuda.synchronize()
-    check_same(ref, Cs, "k-chunked")
+    check_same(ref, Cs, "k-chunked", ps)
     b.close()
 
 
@@ -173,7 +191,7 @@ def test_schedules_identical(oz, persistent, largest_first):
                   largest_first=largest_first)
     b.run()
     torch.cuda.synchronize()
-    check_same(ref, Cs, f"persistent={persistent} largest_first={largest_first}")
+    check_same(ref, Cs, f"persistent={persistent} largest_first={largest_first}", ps)
     b.close()
 
 
@@ -191,5 +209,5 @@ def test_autotuned_plan(oz):
                   largest_first=tune["largest_first"])
     b.run()
     torch.cuda.synchronize()
-    check_same(ref, Cs, "autotuned")
+    check_same(ref, Cs, "autotuned", ps)
     b.close()

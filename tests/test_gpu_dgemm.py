@@ -237,27 +237,40 @@ def test_cpu_baseline_configuration(oz):
     assert r["bound"] <= 1.0
 
 
+@pytest.fixture(scope="module")
+def full8192():
+    # the benchmark configuration's inputs: A, B, C0 U[-1,1) of 8192 x 8192 (host and device)
+    m = 8192
+    rng = np.random.default_rng(17)
+    A = rng.uniform(-1, 1, m * m)
+    B = rng.uniform(-1, 1, m * m)
+    C0 = rng.uniform(-1, 1, m * m)
+    return {"A": A, "B": B, "C0": C0, "dA": to_dev(A), "dB": to_dev(B)}
+
+
 @pytest.mark.slow
-def test_full_size_configuration_sampled(oz):
-    # the benchmark configuration at full size: DGEMM 8192^3 N/N, s = 16, alpha = 1.5,
-    # beta = 0.5, C0 U[-1,1); sampled entries against the exact closed form with the
-    # library's epilogue fma(alpha, v, beta*c) applied exactly on the host
+@pytest.mark.parametrize("ta,tb,s", [("N", "N", 16), ("N", "T", 16), ("T", "N", 16), ("T", "T", 16),
+                                     ("N", "N", 8), ("N", "N", 12)])
+def test_full_size_configuration_sampled(oz, full8192, ta, tb, s):
+    # the benchmark configuration at full size: DGEMM 8192^3, every op(A)/op(B) at s = 16 and
+    # N/N at s = 8 and 12, alpha = 1.5, beta = 0.5, C0 U[-1,1); sampled entries (corners
+    # included) against the exact closed form with the library's epilogue fma(alpha, v, beta*c)
+    # applied exactly on the host
     from gpu_cases import fma
     m = n = k = 8192
-    rng = np.random.default_rng(17)
-    A = rng.uniform(-1, 1, m * k)
-    B = rng.uniform(-1, 1, k * n)
-    C0 = rng.uniform(-1, 1, m * n)
-    log("case full-size dgemm NN 8192^3 s=16 alpha=1.5 beta=0.5")
-    dC = to_dev(C0)
-    oz.gemm("N", "N", m, n, k, 1.5, to_dev(A), m, to_dev(B), k, 0.5, dC, m, 16)
+    F = full8192
+    rng = np.random.default_rng(s * 4 + "NT".index(ta) * 2 + "NT".index(tb))
+    log(f"case full-size dgemm {ta}{tb} 8192^3 s={s} alpha=1.5 beta=0.5")
+    dC = to_dev(F["C0"])
+    oz.gemm(ta, tb, m, n, k, 1.5, F["dA"], m, F["dB"], k, 0.5, dC, m, s)
     C = dC.cpu().numpy()
+    del dC
     rows = rng.integers(0, m, 256)
     cols = rng.integers(0, n, 256)
     rows[:4] = [0, m - 1, 0, m - 1]
     cols[:4] = [0, 0, n - 1, n - 1]
-    v = O.gemm_entries("N", "N", m, n, k, A, m, B, k, 16, rows, cols)
-    want = np.array([fma(1.5, float(x), 0.5 * C0[i + j * m]) for x, i, j in zip(v, rows, cols)])
+    v = O.gemm_entries(ta, tb, m, n, k, F["A"], m, F["B"], k, s, rows, cols)
+    want = np.array([fma(1.5, float(x), 0.5 * F["C0"][i + j * m]) for x, i, j in zip(v, rows, cols)])
     assert np.array_equal(bits(C[rows + cols * m]), bits(want))
 
 

@@ -6,7 +6,7 @@ limits that force multi-tile runs.  Every C must be bit-identical to the oracle 
 the same inputs (the reference's algorithm restated; tests/test_ozaki2.cpp:95-194
 runs one fixed case per behaviour, this covers their combinations).  Random batches
 (oz2_batch_*) of such problems, with random tiles, schedules, order and graph use,
-must reproduce oz2_dgemm bit for bit on every replay."""
+must reproduce the oracle (and oz2_dgemm) bit for bit on every replay."""
 import numpy as np
 import pytest
 import torch
@@ -62,7 +62,7 @@ def test_random_cases_match_oracle(seed):
 def test_random_batches_match_single_calls(seed):
     # batches of 4-24 random problems (every op, s, alpha/beta, k including 0 and
     # k-chunked lengths), random per-problem tiles, schedule, order and graph use,
-    # replayed twice: every problem's C bit-identical to oz2_dgemm with the same s
+    # replayed twice: every problem's C bit-identical to the oracle and to oz2_dgemm with the same s
     from test_gpu_batch import build, check_same, desc, reference
     rng = np.random.default_rng(seed)
     oz = oz2.Ozaki2()
@@ -93,7 +93,7 @@ def test_random_batches_match_single_calls(seed):
                     C.copy_(to_dev(p["C0"]))
                 b.run()
                 torch.cuda.synchronize()
-                check_same(ref, Cs, f"random batch {seed}/{c}")
+                check_same(ref, Cs, f"random batch {seed}/{c}", ps)
             b.close()
     finally:
         oz.close()

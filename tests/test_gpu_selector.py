@@ -9,6 +9,7 @@ import torch
 import oz2
 from conftest import make_mat
 from gpu_cases import flat, log, to_dev
+from oracle import oracle as O
 from oracle import selector as S
 
 pytestmark = pytest.mark.gpu
@@ -226,3 +227,70 @@ def test_trained_selector_gain(oz):
     for r in rows:
         if r[0] == "ints":  # exact integer data needs a handful of moduli, not 15-16
             assert r[4] <= 8 < r[6] and r[5] <= 1.0, r
+
+
+def test_choice_rule_threshold_and_largest_admissible(oz):
+    # the choice rule with weights whose outputs are known: zero weights and output biases b3 give
+    # probabilities sigmoid(b3) for every problem.  Outputs from s = 7 up above the threshold: the
+    # smallest such admissible s (7, or the smallest admissible above it).  Every output below the
+    # threshold: the largest admissible s (24), not moduli_for_bits.
+    rng = np.random.default_rng(4)
+    saved, tau = oz.selector_weights()
+    b3 = oz2.SEL_PARAMS - oz2.SEL_OUTPUTS
+    try:
+        ds = []
+        for (ta, tb, m, n, k, dist) in FEATURE_CASES[1:4]:
+            A, lda, B, ldb = _problem(ta, tb, m, n, k, dist, rng)
+            ds.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, A=to_dev(A), lda=lda, B=to_dev(B), ldb=ldb))
+        W = np.zeros(oz2.SEL_PARAMS, np.float32)
+        W[b3:] = -20.0
+        W[b3 + 5:] = 20.0
+        oz.selector_set_weights(W, 0.5)
+        s, probs, learned = oz.selector_choose(ds, bits=53)
+        for d, sv in zip(ds, s):
+            want = min(v for v in range(7, 25) if O.precision_bits(d["k"], v) > 0)
+            assert sv == want, (d["k"], sv)
+        assert all(learned)
+        W[b3:] = -20.0
+        oz.selector_set_weights(W, 0.5)
+        s, probs, learned = oz.selector_choose(ds, bits=53)
+        assert s == [24] * len(ds) and all(learned) and (probs < 0.5).all()
+    finally:
+        if saved is not None:
+            oz.selector_set_weights(saved, tau)
+
+
+def test_batch_reselect_after_new_contents(oz):
+    # the learned s is fixed when a plan is made: integer data need few moduli; the same buffers then
+    # hold wide-exponent data, which the old s cannot carry to 53 bits.  reselect() chooses again from
+    # the current contents and rebuilds the plan; the rebuilt plan meets the target.
+    from oz2 import check
+    saved, _ = oz.selector_weights()
+    if saved is None:
+        pytest.skip("no trained selector weights are compiled into this build")
+    rng = np.random.default_rng(31)
+    m, n, k = 256, 192, 2000
+    A = to_dev(_gain_matrix("ints", m, k, rng))
+    B = to_dev(_gain_matrix("ints", k, n, rng))
+    C = torch.empty(m * n, dtype=torch.float64, device="cuda")
+    b = oz2.Batch(oz, [dict(ta="N", tb="N", m=m, n=n, k=k, alpha=1.0, A=A, lda=m, B=B, ldb=k, beta=0.0, C=C, ldc=m)],
+                  bits=53, graph=True, select=True)
+    try:
+        s_int = b.moduli()[0]
+        A.copy_(to_dev(_gain_matrix("wide20", m, k, rng)))
+        B.copy_(to_dev(_gain_matrix("wide20", k, n, rng)))
+        hi, lo = check.dd_gemm("N", "N", m, n, k, A, m, B, k)
+        b.run()
+        torch.cuda.synchronize()
+        stale = float(check.ulp_errors(C, hi, lo, m, m).max())
+        assert b.moduli()[0] == s_int
+        assert b.reselect() == 1
+        s_wide = b.moduli()[0]
+        b.run()
+        torch.cuda.synchronize()
+        fresh = float(check.ulp_errors(C, hi, lo, m, m).max())
+        log(f"reselect: ints s={s_int} on wide data {stale:.3g} ulp; reselected s={s_wide} {fresh:.3g} ulp")
+        assert s_wide > s_int and stale > 1.0 and fresh <= 1.0
+        assert b.reselect() == 0  # same contents, same choice
+    finally:
+        b.close()

@@ -179,3 +179,54 @@ def test_accuracy_harness_rows():
         s16 = rows[2]
         assert s16[3] == "16" and float(s16[7]) == 1.0 and float(s16[5]) <= 0.5  # correctly rounded at s = 16
         assert float(rows[1][5]) > 1.0  # s = 6 keeps too few bits
+
+
+@pytest.mark.parametrize("tile", [256, 128, 64, 256128])
+def test_unit_gemm_against_int64_brute_force(tile):
+    # the INT8 GEMM with its residue epilogue (the reference's oneMKL gemm_batch, hpp:1029-1033):
+    # random int8 planes over the whole int8 range with ragged m, n, k (zero padding up to the macro
+    # tile and to kp), every residue byte against (sum of int64 products) mod m_t
+    rng = np.random.default_rng(tile)
+    tm, tn = (256, 128) if tile == 256128 else (tile, tile)
+    for m, n, k, s in ((tm + 37, 2 * tn - 5, 300, 16), (tm - 3, tn + 1, 1, 5), (2 * tm + 1, 3 * tn - 1, 2500, 24)):
+        mpad, npad = -(-m // tm) * tm, -(-n // tn) * tn
+        kp = -(-k // 128) * 128
+        ldr = mpad + 16
+        log(f"unit gemm tile={tile} m={m} n={n} k={k} s={s} (mpad={mpad} npad={npad} kp={kp})")
+        a = np.zeros((s, mpad, kp), dtype=np.int8)
+        b = np.zeros((s, npad, kp), dtype=np.int8)
+        a[:, :m, :k] = rng.integers(-128, 128, (s, m, k))
+        b[:, :n, :k] = rng.integers(-128, 128, (s, n, k))
+        # the first planes at the extremes: all -128 (largest |sum|) and alternating signs
+        a[0, :m, :k] = -128
+        b[0, :n, :k] = -128
+        ia, ib = plane_index(mpad, kp).reshape(-1), plane_index(npad, kp).reshape(-1)
+        pa = np.zeros((s, mpad * kp), dtype=np.int8)
+        pb = np.zeros((s, npad * kp), dtype=np.int8)
+        pa[:, ia] = a.reshape(s, -1)
+        pb[:, ib] = b.reshape(s, -1)
+        R = torch.full((s * npad * ldr,), 0xEE, dtype=torch.uint8, device="cuda")
+        oz2.unit_gemm(to_dev(pa.reshape(-1)), to_dev(pb.reshape(-1)), mpad, npad, kp, s, tile, R, ldr)
+        torch.cuda.synchronize()
+        got = R.cpu().numpy().reshape(s, npad, ldr)
+        mods = O.moduli(s)
+        for t in range(s):
+            # |sums| < 2^30: the float64 products and sums of these integers are exact
+            prod = (a[t].astype(np.float64) @ b[t].astype(np.float64).T).astype(np.int64)
+            want = np.mod(prod, int(mods[t])).astype(np.uint8)
+            assert np.array_equal(got[t, :, :mpad], want.T), f"tile {tile} plane {t} (m_t = {mods[t]})"
+            assert (got[t, :, mpad:] == 0xEE).all(), "bytes past mpad in each column must stay untouched"
+
+
+def test_unit_gemm_argument_errors():
+    P = torch.zeros(1 << 16, dtype=torch.int8, device="cuda")
+    R = torch.zeros(1 << 16, dtype=torch.uint8, device="cuda")
+    for args in ((100, 128, 128, 2, 128, 128),     # mpad not a tile multiple
+                 (128, 128, 100, 2, 128, 128),     # kp not a multiple of 128
+                 (128, 128, 65536, 2, 128, 128),   # more than one k-chunk
+                 (128, 128, 128, 2, 128, 120),     # ldr below mpad
+                 (128, 128, 128, 25, 128, 128),    # num_moduli
+                 (128, 128, 128, 2, 96, 128)):     # tile code
+        mpad, npad, kp, s, tile, ldr = args
+        with pytest.raises(oz2.OZ2InvalidArgument):
+            oz2.unit_gemm(P, P, mpad, npad, kp, s, tile, R, ldr)
