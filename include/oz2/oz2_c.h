@@ -50,6 +50,19 @@ int oz2_zgemm(oz2_handle h, int ta, int tb, int64_t m, int64_t n, int64_t k, con
               int64_t lda, const double* B, int64_t ldb, const double* beta, double* C, int64_t ldc, int num_moduli,
               void* const* deps, int ndeps, void* done);
 
+/* oz2_dgemm with nothing allocated (no reference counterpart; for callers that cannot allocate device
+ * memory: work under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, semantics and results as oz2_dgemm; every intermediate lives in a fixed device array of
+ * OZ2_NOALLOC_SCRATCH_BYTES that each GPU holds from the library's load until the process ends, and C is
+ * cut into panels whose intermediates fit it.  A product whose smallest panels (64 x 64 of C, all s
+ * residue planes of their k) do not fit returns OZ2_INVALID_ARGUMENT before anything is enqueued.  No
+ * stage times are recorded.  All calls on one GPU share its array: calls that overlap on one GPU (two
+ * handles on two streams, two threads) race; keep them on one in-order stream. */
+#define OZ2_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int oz2_dgemm_noalloc(oz2_handle h, int ta, int tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                      int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int num_moduli,
+                      void* const* deps, int ndeps, void* done);
+
 /* Ozaki2::precision_bits (hpp:691), precision_bits_complex (hpp:762), moduli_for_bits (hpp:693-700).
  * precision_bits returns tk >= 1, or -1 with the error code in *err (may be NULL). */
 int oz2_precision_bits(int64_t k, int num_moduli, int* err);

@@ -125,7 +125,10 @@ extern "C" {
 int oz2_create(void* stream, oz2_handle* out) {
     return guard([&] {
         if (!out) throw std::invalid_argument("oz2: null output pointer");
-        *out = new oz2_engine(static_cast<hipStream_t>(stream));
+        auto* h = new oz2_engine(static_cast<hipStream_t>(stream));
+        // the device's scratch of oz2_dgemm_noalloc is looked up here, at set-up
+        oz2::capi::noalloc_attach(h->eng.device());
+        *out = h;
     });
 }
 

@@ -1,4 +1,4 @@
-// capi.hpp - internals shared by the C ABI's source files:
+// capi.hpp - internals shared by the C ABI's source files (c_api.cpp, noalloc.hip):
 // the handle, the exception -> return code translation and argument helpers.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -74,6 +74,9 @@ inline void signal(hipStream_t st, void* done) {
     if (done && hipEventRecord(static_cast<hipEvent_t>(done), st) != hipSuccess)
         throw std::runtime_error("oz2: hipEventRecord failed");
 }
+
+// oz2_create calls this with the handle's device current (noalloc.hip)
+void noalloc_attach(int dev);
 
 }  // namespace capi
 }  // namespace oz2

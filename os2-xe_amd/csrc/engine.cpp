@@ -189,16 +189,26 @@ size_t Engine::workspace_for(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int 
 
 void Engine::gemm(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
                   const double* B, int64_t ldb, double beta, double* C, int64_t ldc, int s) {
-    run(ta, tb, m, n, k, alpha, 0.0, A, lda, B, ldb, beta, 0.0, C, ldc, s, false);
+    run(ta, tb, m, n, k, alpha, 0.0, A, lda, B, ldb, beta, 0.0, C, ldc, s, false, nullptr, 0);
 }
 
 void Engine::zgemm(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, double ai, const double* A, int64_t lda,
                    const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s) {
-    run(ta, tb, m, n, k, ar, ai, A, lda, B, ldb, br, bi, C, ldc, s, true);
+    run(ta, tb, m, n, k, ar, ai, A, lda, B, ldb, br, bi, C, ldc, s, true, nullptr, 0);
+}
+
+void Engine::gemm_fixed(std::byte* ws, size_t ws_bytes, Op ta, Op tb, int64_t m, int64_t n, int64_t k, double alpha,
+                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                        int64_t ldc, int s) {
+    if (!ws) throw std::runtime_error("oz2: no fixed workspace on this device");
+    run(ta, tb, m, n, k, alpha, 0.0, A, lda, B, ldb, beta, 0.0, C, ldc, s, false, ws, ws_bytes);
 }
 
 void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, double ai, const double* A, int64_t lda,
-                 const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s, bool cplx) {
+                 const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s, bool cplx,
+                 std::byte* fixed_ws, size_t fixed_bytes) {
+    // a call into a fixed workspace creates nothing: no stage events either
+    const bool timed = profiling_ && !fixed_ws;
     if (!accumulate_) stages_.clear();
     ++calls_;
     // validation (hpp:745-748)
@@ -218,28 +228,37 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
     const TabSet* tab = dtabs_ + s;
 
     const CallShape cs = call_shape(ta, tb, m, n, k, s, cplx);
-    const TileInfo ti = plan_for_call(cs, budget());
+    if (fixed_ws) {
+        // the smallest panels the planner can take: 64 x 64 (or all of a shorter m, n)
+        const size_t need = layout(cs, std::min<int64_t>(m, 64), std::min<int64_t>(n, 64)).total;
+        if (need > fixed_bytes)
+            throw std::invalid_argument("oz2: product too large for the fixed workspace: its smallest panels need " +
+                                        std::to_string((need + (size_t{1} << 20) - 1) >> 20) + " MiB, it holds " +
+                                        std::to_string(fixed_bytes >> 20) + " MiB");
+    }
+    const TileInfo ti = plan_for_call(cs, fixed_ws ? fixed_bytes : budget());
     const int64_t mb = ti.mb, nb = ti.nb;
     const Layout L = layout(cs, mb, nb);
     const int64_t kp = L.kp, kch = L.kch, klen = L.klen;
-    ensure_workspace(L.total);
+    if (!fixed_ws) ensure_workspace(L.total);
     tiling_ = ti;
     tile_ = L.tile;
+    std::byte* const ws = fixed_ws ? fixed_ws : ws_;
 
-    int32_t* D = reinterpret_cast<int32_t*>(ws_ + L.oD);
-    int32_t* E = reinterpret_cast<int32_t*>(ws_ + L.oE);
-    uint64_t* PA = reinterpret_cast<uint64_t*>(ws_ + L.oPA);
-    uint64_t* PB = reinterpret_cast<uint64_t*>(ws_ + L.oPB);
-    int8_t* Ap = reinterpret_cast<int8_t*>(ws_ + L.oA);
-    int8_t* Bp = reinterpret_cast<int8_t*>(ws_ + L.oB);
-    uint8_t* R = reinterpret_cast<uint8_t*>(ws_ + L.oR);
+    int32_t* D = reinterpret_cast<int32_t*>(ws + L.oD);
+    int32_t* E = reinterpret_cast<int32_t*>(ws + L.oE);
+    uint64_t* PA = reinterpret_cast<uint64_t*>(ws + L.oPA);
+    uint64_t* PB = reinterpret_cast<uint64_t*>(ws + L.oPB);
+    int8_t* Ap = reinterpret_cast<int8_t*>(ws + L.oA);
+    int8_t* Bp = reinterpret_cast<int8_t*>(ws + L.oB);
+    uint8_t* R = reinterpret_cast<uint8_t*>(ws + L.oR);
     const int64_t cw = cplx ? 2 : 1;  // doubles per element
 
-    record(Stage::Begin);
+    if (timed) record(Stage::Begin);
     // scale: exponents of the rows of op(A) and the columns of op(B) (hpp:974-989)
     k::scale(stream_, A, lda, m, k, ta != Op::N, cplx, tk, PA, D);
     k::scale(stream_, B, ldb, n, k, tb == Op::N, cplx, tk, PB, E);
-    record(Stage::Scale);
+    if (timed) record(Stage::Scale);
 
     for (int64_t j0 = 0; j0 < n; j0 += nb) {
         const int64_t nbj = std::min(nb, n - j0);
@@ -261,7 +280,7 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
         sb.wide = wide;
         sb.tab = tab;
         k::split(stream_, sb);
-        record(Stage::Split);
+        if (timed) record(Stage::Split);
         for (int64_t i0 = 0; i0 < m; i0 += mb) {
             const int64_t mbi = std::min(mb, m - i0);
             if (tiling_.pm > 1 || j0 == 0) {
@@ -283,7 +302,7 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
                 sa.wide = wide;
                 sa.tab = tab;
                 k::split(stream_, sa);
-                record(Stage::Split);
+                if (timed) record(Stage::Split);
             }
             k::GemmArgs ga{};
             ga.A = Ap;
@@ -317,7 +336,7 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
                     k::igemm_residue(stream_, tile_, g1);
                 }
             }
-            record(Stage::Gemm);
+            if (timed) record(Stage::Gemm);
             k::CrtArgs ca{};
             ca.R = R;
             ca.plane_r = L.planeR;
@@ -339,7 +358,7 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
             ca.L16 = 2 * p.NL;
             ca.tab = tab;
             k::crt(stream_, ca);
-            record(Stage::Crt);
+            if (timed) record(Stage::Crt);
         }
     }
     hip_check(hipGetLastError(), "kernel launch");

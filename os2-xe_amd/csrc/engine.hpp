@@ -66,6 +66,12 @@ public:
 
     // Workspace a call would need (no allocation); used to pre-size before graph capture.
     size_t workspace_for(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx);
+    // gemm() with all intermediates in the caller's device buffer ws of ws_bytes: the panels
+    // are planned to fit it (std::invalid_argument if even the smallest do not), nothing is
+    // allocated or created, and no stage times are recorded
+    void gemm_fixed(std::byte* ws, size_t ws_bytes, Op ta, Op tb, int64_t m, int64_t n, int64_t k, double alpha,
+                    const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+                    int s);
 
     // shared with batch plans
     const TabSet* device_tables() const { return dtabs_; }
@@ -77,7 +83,8 @@ public:
 private:
     enum class Stage { Begin, Scale, Split, Gemm, Crt };
     void run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, double ai, const double* A, int64_t lda,
-             const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s, bool cplx);
+             const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s, bool cplx,
+             std::byte* fixed_ws, size_t fixed_bytes);
     void record(Stage st);
     CallShape call_shape(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx) const;
     // plan_tiling, remembered for the last shape and limit

@@ -25,6 +25,7 @@ if not os.path.exists(LIB_PATH):
 _lib = ctypes.CDLL(LIB_PATH)
 
 N, T, C = 0, 1, 2
+NOALLOC_SCRATCH_BYTES = 64 << 20  # OZ2_NOALLOC_SCRATCH_BYTES in oz2_c.h
 _TRANS = {"N": 0, "T": 1, "C": 2, 0: 0, 1: 1, 2: 2}
 
 _i64, _int, _dbl, _vp, _sz = ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
@@ -82,6 +83,7 @@ _lib.oz2_dgemm.argtypes = [_vp, _int, _int, _i64, _i64, _i64, _dbl, _vp, _i64, _
                            _P(_vp), _int, _vp]
 _lib.oz2_zgemm.argtypes = [_vp, _int, _int, _i64, _i64, _i64, _P(_dbl), _vp, _i64, _vp, _i64, _P(_dbl), _vp, _i64,
                            _int, _P(_vp), _int, _vp]
+_lib.oz2_dgemm_noalloc.argtypes = _lib.oz2_dgemm.argtypes
 _lib.oz2_precision_bits.argtypes = [_i64, _int, _P(_int)]
 _lib.oz2_precision_bits_complex.argtypes = [_i64, _int, _P(_int)]
 _lib.oz2_moduli_for_bits.argtypes = [_i64, _int]
@@ -132,7 +134,7 @@ _lib.oz2_last_error.restype = ctypes.c_char_p
 _lib.oz2_version.restype = ctypes.c_char_p
 
 EXPORTS = [
-    "oz2_create", "oz2_destroy", "oz2_set_stream", "oz2_dgemm", "oz2_zgemm", "oz2_precision_bits",
+    "oz2_create", "oz2_destroy", "oz2_set_stream", "oz2_dgemm", "oz2_zgemm", "oz2_dgemm_noalloc", "oz2_precision_bits",
     "oz2_precision_bits_complex", "oz2_moduli_for_bits", "oz2_set_workspace_limit", "oz2_set_resident_hint",
     "oz2_release_workspace", "oz2_reserve_workspace", "oz2_workspace_query", "oz2_stage_times", "oz2_last_tiling",
     "oz2_set_tuning", "oz2_get_tuning", "oz2_set_profiling", "oz2_profile_begin", "oz2_profile_end", "oz2_moduli",
@@ -306,6 +308,12 @@ class Ozaki2:
         dep = (_vp * max(1, len(deps)))(*[int(d) for d in deps])
         _check(_lib.oz2_dgemm(self._h, _TRANS.get(ta, -1), _TRANS.get(tb, -1), m, n, k, float(alpha), _ptr(A), lda, _ptr(B), ldb,
                               float(beta), _ptr(C), ldc, num_moduli, dep, len(deps), done))
+
+    def gemm_noalloc(self, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, num_moduli, deps=(), done=None):
+        """gemm() with every intermediate in the library's fixed per-GPU scratch (oz2_dgemm_noalloc)"""
+        dep = (_vp * max(1, len(deps)))(*[int(d) for d in deps])
+        _check(_lib.oz2_dgemm_noalloc(self._h, _TRANS.get(ta, -1), _TRANS.get(tb, -1), m, n, k, float(alpha), _ptr(A), lda,
+                                      _ptr(B), ldb, float(beta), _ptr(C), ldc, num_moduli, dep, len(deps), done))
 
     def zgemm(self, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, num_moduli, deps=(), done=None):
         al = (_dbl * 2)(complex(alpha).real, complex(alpha).imag)
