@@ -65,6 +65,8 @@ def parse():
                     choices=["dgemm8192", "zgemm4096", "wide4000", "ksplit", "batch"],
                     help="dgemm8192 (default line), or one of the other measured configurations")
     ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--overlap", type=int, default=-1,
+                    help="column blocks overlapped over two streams (Tuning::overlap_blocks): -1 automatic, 1 none")
     return ap.parse_args()
 
 
@@ -134,6 +136,15 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
     t_plain = timed(lambda: plain(s_mfb))
     ulp_plain = max_ulp()
     descs = [dict(p, num_moduli=0) for p in probs]
+    # the selector's own cost (exponent features of every problem + the network, two stream
+    # synchronizations): paid when a plan is made or re-selected, not by its replays
+    oz.selector_choose(descs, bits=bits)
+    torch.cuda.synchronize()
+    t_sel = float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oz.selector_choose(descs, bits=bits)
+        t_sel = min(t_sel, time.perf_counter() - t0)
     tune = oz.batch_autotune(descs, bits=bits, select=True)
     b = oz2.Batch(oz, descs, bits=bits, graph=True, select=True, tiles=tune["tiles"],
                   persistent=tune["persistent"], largest_first=tune["largest_first"])
@@ -174,11 +185,16 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
                                                           for t in (64, 128, 256, 256128)},
                                                 "tune_ms_no_graph": round(tune["ms"], 3)}},
         "speedup_vs_plain": round(t_plain / t_batch, 3),
+        "selector": {"ms": round(t_sel * 1e3, 3),
+                     "what": "features of all problems + MLP + choice (oz2_selector_choose), host wall time; paid "
+                             "once per plan (creation or oz2_batch_reselect), not per replay",
+                     "learned_plus_selection_ms": round((t_batch + t_sel) * 1e3, 3)},
     }
     if s_eq is not None:
         out["plain_equal_max_ulp"] = {"s": s_eq, "max_ulp": ulp_eq, "tflops": round(flops / t_eq / 1e12, 3),
                                       "ms": round(t_eq * 1e3, 3)}
         out["speedup_at_equal_max_ulp"] = round(t_eq / t_batch, 3)
+        out["speedup_at_equal_max_ulp_selection_every_run"] = round(t_eq / (t_batch + t_sel), 3)
     return out
 
 
@@ -245,6 +261,9 @@ def zgemm_line(args, torch):
                         "kernel": "igemm_residue_kernel (2*m*n*k*3s int8 ops per launch)", "gemm_ms": round(gemm_ms, 4)},
            "stage_ms": {kk: round(v / max(1, calls), 4) for kk, v in st.as_dict().items()}}
     out["hbm_stages"] = hbm_stages(m, n, k, s, True, True, out["stage_ms"])
+    if args.no_accuracy:
+        oz.close()
+        return out
     # accuracy with alpha=1, beta=0 against double-double (complex)
     dC1 = torch.empty(2 * ldc * n, dtype=torch.float64, device="cuda")
     oz.zgemm("C", "N", m, n, k, 1.0, dA, lda, dB, ldb, 0.0, dC1, ldc, s)
@@ -559,6 +578,7 @@ def main():
     dC = dC0.clone()
     oz = oz2.Ozaki2()
     oz.set_profiling(True)
+    oz.set_tuning(overlap_blocks=args.overlap)
 
     for _ in range(args.warmup):
         oz.gemm("N", "N", m, n, k, alpha, dA, m, dB, k, beta, dC, m, s)

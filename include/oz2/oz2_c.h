@@ -30,7 +30,10 @@ enum { OZ2_TRANS_N = 0, OZ2_TRANS_T = 1, OZ2_TRANS_C = 2 }; /* Trans{N,T,C}, hpp
 
 typedef struct { double scale_ms, split_ms, gemm_ms, crt_ms; } oz2_stage_times_t; /* StageTimes, hpp:36-39 */
 typedef struct { int64_t mb, nb, pm, pn, k_chunks; size_t workspace_bytes; } oz2_tile_info_t; /* TileInfo, hpp:47-52 */
-typedef struct { int batched_gemm, pad_lda, pow2_tiles, gemm_tile; } oz2_tuning_t; /* Tuning, hpp:41-45 (+ gemm_tile) */
+/* Tuning, hpp:41-45, plus gemm_tile (-1 auto, else 256 / 128 / 64 / 256128) and overlap_blocks (column blocks
+ * of each panel pair whose B split and CRT run on a second stream beside the next block's GEMM: -1 auto,
+ * 0 or 1 none, up to 16) */
+typedef struct { int batched_gemm, pad_lda, pow2_tiles, gemm_tile, overlap_blocks; } oz2_tuning_t;
 
 /* Ozaki2(sycl::queue&) / ~Ozaki2() (hpp:680-687).  stream: a hipStream_t, NULL = default stream. */
 int oz2_create(void* stream, oz2_handle* out);
@@ -202,6 +205,15 @@ int oz2_unit_crt(void* stream, const uint8_t* R, int64_t ldr, int64_t mb, int64_
  * multiples of it); kp a multiple of 128 up to 65408 (one k-chunk); ldr >= mpad, a multiple of 16. */
 int oz2_unit_gemm(void* stream, const int8_t* A, const int8_t* B, int64_t mpad, int64_t npad, int64_t kp,
                   int num_moduli, int tile, uint8_t* R, int64_t ldr);
+
+/* Ozaki Scheme I with L slices, the reference's comparison baseline of its accuracy benchmark
+ * (oz1::Ozaki1::gemm, benchmarks/ozaki1_gpu.hpp:100-151): C = A*B (op N, alpha 1, beta 0), rows of A and
+ * columns of B scaled to tk = 7L - 2 bits, L balanced base-128 digits, the L(L+1)/2 digit products with
+ * i + j < L as INT8 GEMMs and a double-double Horner over the anti-diagonals.  2 <= L <= 9 and
+ * k * 4096 < 2^31 (the reference's limits and "oz1: ..." messages).  Enqueued on the handle's stream,
+ * in its workspace. */
+int oz2_ozaki1_dgemm(oz2_handle h, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda, const double* B,
+                     int64_t ldb, double* C, int64_t ldc, int L);
 
 const char* oz2_last_error(void);
 const char* oz2_version(void);

@@ -41,6 +41,7 @@ struct Tuning {
     bool pad_lda = true;
     bool pow2_tiles = true;
     int gemm_tile = -1;  // MI355X extension: -1 auto, 256 / 128 / 64 / 256128 (256 x 128) forces the INT8 GEMM macro tile
+    int overlap_blocks = -1;  // MI355X extension: column blocks overlapped over two streams, -1 auto, 0 or 1 none
 };
 
 struct TileInfo {
@@ -124,7 +125,7 @@ public:
         return tiling_;
     }
     void set_tuning(const Tuning& t) {
-        const oz2_tuning_t c{t.batched_gemm, t.pad_lda, t.pow2_tiles, t.gemm_tile};
+        const oz2_tuning_t c{t.batched_gemm, t.pad_lda, t.pow2_tiles, t.gemm_tile, t.overlap_blocks};
         detail::check(oz2_set_tuning(h_, &c));
         tune_ = t;
     }

@@ -246,6 +246,9 @@ int oz2_set_tuning(oz2_handle h, const oz2_tuning_t* t) {
         u.pad_lda = t->pad_lda != 0;
         u.pow2_tiles = t->pow2_tiles != 0;
         u.gemm_tile = t->gemm_tile;
+        if (t->overlap_blocks < -1 || t->overlap_blocks > oz2::kMaxOverlapBlocks)
+            throw std::invalid_argument("oz2: overlap_blocks must be -1 (automatic) or 0 .. 16");
+        u.overlap_blocks = t->overlap_blocks;
         h->eng.set_tuning(u);
     });
 }
@@ -255,7 +258,7 @@ int oz2_get_tuning(oz2_handle h, oz2_tuning_t* out) {
         need(h);
         if (!out) throw std::invalid_argument("oz2: null output pointer");
         const oz2::Tuning& u = h->eng.tuning();
-        *out = oz2_tuning_t{u.batched_gemm, u.pad_lda, u.pow2_tiles, u.gemm_tile};
+        *out = oz2_tuning_t{u.batched_gemm, u.pad_lda, u.pow2_tiles, u.gemm_tile, u.overlap_blocks};
     });
 }
 
@@ -549,6 +552,14 @@ int oz2_unit_gemm(void* stream, const int8_t* A, const int8_t* B, int64_t mpad, 
         a.tab = unit_tables() + num_moduli;
         oz2::k::igemm_residue(static_cast<hipStream_t>(stream), t, a);
         if (hipGetLastError() != hipSuccess) throw std::runtime_error("oz2: gemm launch failed");
+    });
+}
+
+int oz2_ozaki1_dgemm(oz2_handle h, int64_t m, int64_t n, int64_t k, const double* A, int64_t lda, const double* B,
+                     int64_t ldb, double* C, int64_t ldc, int L) {
+    return guard([&] {
+        need(h);
+        h->eng.ozaki1(m, n, k, A, lda, B, ldb, C, ldc, L);
     });
 }
 

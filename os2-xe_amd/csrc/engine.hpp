@@ -28,7 +28,9 @@ struct Tuning {
     bool pad_lda = true;       // stagger plane strides that are multiples of 64 KiB by 4 KiB (engine.cpp)
     bool pow2_tiles = true;    // include power-of-two tile candidates
     int gemm_tile = -1;        // -1 auto, else a tile code: 256 / 128 / 64 (square) or 256128 (256 x 128)
+    int overlap_blocks = -1;   // column blocks per panel pair overlapped over two streams: -1 auto, 0 or 1 none
 };
+inline constexpr int kMaxOverlapBlocks = 16;
 
 enum class Op { N = 0, T = 1, C = 2 };
 
@@ -73,6 +75,11 @@ public:
                     const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
                     int s);
 
+    // Ozaki Scheme I with L slices (the reference's comparison baseline oz1::Ozaki1::gemm,
+    // benchmarks/ozaki1_gpu.hpp:100-151): C = A B, op N, alpha 1, beta 0, on this library's kernels
+    void ozaki1(int64_t m, int64_t n, int64_t k, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+                int64_t ldc, int L);
+
     // shared with batch plans
     const TabSet* device_tables() const { return dtabs_; }
     const Plan& plan_for(int s, int64_t k);
@@ -81,11 +88,20 @@ public:
     int compute_units() const { return cus_; }
 
 private:
-    enum class Stage { Begin, Scale, Split, Gemm, Crt };
+    enum class Stage { Scale, Split, Gemm, Crt };
+    struct Mark {
+        Stage st;
+        size_t ev;  // events_[ev], events_[ev + 1]: begin and end on the stream the stage ran on
+    };
     void run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, double ai, const double* A, int64_t lda,
              const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s, bool cplx,
              std::byte* fixed_ws, size_t fixed_bytes);
-    void record(Stage st);
+    hipEvent_t timing_event(size_t idx);
+    size_t stage_begin(Stage st, hipStream_t on);
+    void stage_end(size_t mark, hipStream_t on);
+    hipEvent_t sync_event();  // from a pool reused by every call (a wait binds the record before it)
+    void order_after(hipStream_t to, hipStream_t from);
+    int overlap_blocks() const;  // column blocks per panel pair (Tuning::overlap_blocks; automatic: 1)
     CallShape call_shape(Op ta, Op tb, int64_t m, int64_t n, int64_t k, int s, bool cplx) const;
     // plan_tiling, remembered for the last shape and limit
     TileInfo plan_for_call(const CallShape& c, size_t limit);
@@ -109,8 +125,11 @@ private:
     size_t memo_limit_ = 0;
     TileInfo memo_ti_{};
     k::Tile tile_ = k::Tile::T256;
-    std::vector<hipEvent_t> events_;
-    std::vector<std::pair<Stage, size_t>> stages_;
+    hipStream_t aux_ = nullptr;  // second stream of the column-block overlap
+    std::vector<hipEvent_t> events_;        // timing events
+    std::vector<Mark> marks_;
+    std::vector<hipEvent_t> sync_events_;   // ordering events between the two streams
+    size_t sync_used_ = 0;
     bool accumulate_ = false;
     int64_t calls_ = 0;
     int cus_ = 256;

@@ -148,8 +148,27 @@ struct TileShape {
     static constexpr int SMEM = NS * STAGE;
 };
 
-// one BM x BN macro tile of one plane: k loop and residue epilogue; smem holds NS stages
-template <int BM, int BN, int NS>
+// int32 sums of one wave tile stored as they are: accumulator v of lane l holds row
+// 8 (v / 4) + 4 (l / 32) + v % 4 and column l % 32 of its 32 x 32 block, so each lane
+// stores four 16-byte runs of 4 rows per block
+template <int FM, int FN>
+__device__ __forceinline__ void int32_epilogue(const GemmArgs& p, v16i (&acc)[FM][FN], uint32_t plane, int64_t row0,
+                                               int64_t col0, int lane) {
+    const int fr = lane & 31, fc = lane >> 5;
+    int32_t* Rp = p.R32 + static_cast<int64_t>(plane) * p.plane_r;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const v4i v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
+                *reinterpret_cast<v4i*>(Rp + (col0 + j * 32 + fr) * p.ldr + row0 + i * 32 + 8 * q + 4 * fc) = v;
+            }
+}
+
+// one BM x BN macro tile of one plane: k loop and residue (or int32) epilogue; smem holds NS stages
+template <int BM, int BN, int NS, bool I32 = false>
 __device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uint32_t tm, uint32_t tn, v4i* smem) {
     constexpr int WTM = BM / 2, WTN = BN / 2;
     constexpr int FM = WTM / 32, FN = WTN / 32;
@@ -171,7 +190,8 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uin
     const int8_t* Ag = p.A + static_cast<int64_t>(plane) * p.plane_a +
                        ((static_cast<int64_t>(tm) * (BM / 16) + wave * LA) * kblocks + p.k0 / KT) * 1024 + lane * 16;
     const int8_t* Bg = p.B + static_cast<int64_t>(plane) * p.plane_b +
-                       ((static_cast<int64_t>(tn) * (BN / 16) + wave * LB) * kblocks + p.k0 / KT) * 1024 + lane * 16;
+                       ((static_cast<int64_t>(tn) * (BN / 16) + wave * LB) * kblocks + (p.k0 + p.kb_extra) / KT) * 1024 +
+                       lane * 16;
     const int64_t bstride = kblocks * 1024;  // next 16-row block
     char* sbase = reinterpret_cast<char*>(smem);
     // LDS byte offsets of this lane's fragments (row-major 64-byte rows; the
@@ -261,6 +281,11 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uin
 #undef OZ2_HALF
 #undef OZ2_WAIT
     __builtin_amdgcn_s_waitcnt(vm_wait(0));  // no DMA may still target LDS
+    if constexpr (I32) {
+        int32_epilogue<FM, FN>(p, acc, plane, static_cast<int64_t>(tm) * BM + wm * WTM,
+                               static_cast<int64_t>(tn) * BN + wn * WTN, lane);
+        return;
+    }
     __syncthreads();                          // every wave is done with the stage buffers
     residue_epilogue<FM, FN>(p, acc, plane, static_cast<int64_t>(tm) * BM + wm * WTM,
                              static_cast<int64_t>(tn) * BN + wn * WTN, sbase + wave * (WTM * WTN), lane);
@@ -273,6 +298,15 @@ __global__ void __launch_bounds__(256, MINB) igemm_residue_kernel(GemmArgs p) {
     const uint32_t tiles = static_cast<uint32_t>(p.tiles_m) * static_cast<uint32_t>(p.tiles_n);
     tile_coords(p, plane_xcd_remap(blockIdx.x, tiles, gridDim.x), plane, tm, tn);
     gemm_tile<BM, BN, NS>(p, plane, tm, tn, smem);
+}
+
+template <int BM, int BN, int MINB, int NS>
+__global__ void __launch_bounds__(256, MINB) igemm_int32_kernel(GemmArgs p) {
+    __shared__ v4i smem[TileShape<BM, BN, NS>::SMEM / 16];
+    uint32_t plane, tm, tn;
+    const uint32_t tiles = static_cast<uint32_t>(p.tiles_m) * static_cast<uint32_t>(p.tiles_n);
+    tile_coords(p, plane_xcd_remap(blockIdx.x, tiles, gridDim.x), plane, tm, tn);
+    gemm_tile<BM, BN, NS, true>(p, plane, tm, tn, smem);
 }
 
 // Grouped forms over many problems; owner[w] is the item (problem, k-chunk) of
@@ -361,6 +395,20 @@ void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* count
     hipLaunchKernelGGL((igemm_grouped_blocks_kernel<BM, BN, MINB, NS>), dim3(g.blocks), dim3(256), 0, st, items, \
                        g.d_start, g.d_owner)
     switch (t) {
+        case Tile::T256: OZ2_LAUNCH(256, 256, 1, 4); break;
+        case Tile::T128: OZ2_LAUNCH(128, 128, 2, 4); break;
+        case Tile::T64: OZ2_LAUNCH(64, 64, 4, 4); break;
+        case Tile::T256x128: OZ2_LAUNCH(256, 128, 2, 3); break;
+    }
+#undef OZ2_LAUNCH
+}
+
+void igemm_int32(hipStream_t st, Tile tile, const GemmArgs& a) {
+    const uint32_t nblk = static_cast<uint32_t>(a.tiles_m) * a.tiles_n * a.nplanes;
+    if (nblk == 0 || a.klen <= 0) return;
+#define OZ2_LAUNCH(BM, BN, MINB, NS) \
+    hipLaunchKernelGGL((igemm_int32_kernel<BM, BN, MINB, NS>), dim3(nblk), dim3(256), 0, st, a)
+    switch (tile) {
         case Tile::T256: OZ2_LAUNCH(256, 256, 1, 4); break;
         case Tile::T128: OZ2_LAUNCH(128, 128, 2, 4); break;
         case Tile::T64: OZ2_LAUNCH(64, 64, 4, 4); break;

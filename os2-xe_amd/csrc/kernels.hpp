@@ -78,8 +78,13 @@ struct GemmArgs {
     int plane0;           // global index of this launch's first plane: its modulus is (plane0 + plane) mod s
     int s;
     const TabSet* tab;    // device tables of this s
+    int64_t kb_extra;     // op(B) reads k range k0 + kb_extra .. (0 for Scheme II)
+    int32_t* R32;         // igemm_int32: sums [nplanes][npad_c][ldr] as int32 (R unused)
 };
 void igemm_residue(hipStream_t st, Tile tile, const GemmArgs& a);
+// the same GEMM with the int32 sums stored as they are (|sums| < 2^31 is the caller's to keep):
+// the digit products of Ozaki Scheme I
+void igemm_int32(hipStream_t st, Tile tile, const GemmArgs& a);
 
 // ---- CRT + alpha/beta epilogue -------------------------------------------------
 struct CrtArgs {
@@ -99,6 +104,32 @@ struct CrtArgs {
     const TabSet* tab;    // device tables of this s
 };
 void crt(hipStream_t st, const CrtArgs& a);
+
+// ---- Ozaki Scheme I (the accuracy benchmark's comparison baseline) --------------------
+// L balanced base-128 digits of RNE(op(X)[r][p] 2^ex[r]) per element into one extended
+// plane of rows_pad x (L kp) bytes (blocked as plane_offset with L kp columns): digit i
+// (most significant first) at k-slot i, or L - 1 - i with rev
+struct Oz1SliceArgs {
+    const double* X;
+    int64_t ldx, rows, rows_pad, k, kp;
+    const int32_t* ex;
+    int8_t* P;
+    bool kcontig, rev;
+};
+void oz1_slices(hipStream_t st, const Oz1SliceArgs& a, int L);
+// Horner step of anti-diagonal d over the int32 chunk sums Ct[c][j][ldt] (m x n entries)
+struct Oz1HornerArgs {
+    const int32_t* Ct;
+    int64_t ldt, chunk;   // column stride and chunk stride of Ct (elements)
+    int nchunks;
+    double *Th, *Tl;      // running double-double T (m x n)
+    bool first, last;
+    int shift;            // 7 (L - 1)
+    const int32_t *D, *E;
+    double* C;
+    int64_t m, n, ldc;
+};
+void oz1_horner(hipStream_t st, const Oz1HornerArgs& a);
 
 // ---- C = beta * C for k == 0 or alpha == 0 --------------------------------------
 void scale_c(hipStream_t st, double* C, int64_t ldc, int64_t m, int64_t n, double br, double bi, bool cplx);

@@ -50,7 +50,8 @@ class TileInfo(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("batched_gemm", _int), ("pad_lda", _int), ("pow2_tiles", _int), ("gemm_tile", _int)]
+    _fields_ = [("batched_gemm", _int), ("pad_lda", _int), ("pow2_tiles", _int), ("gemm_tile", _int),
+                ("overlap_blocks", _int)]
 
 
 class OZ2Error(Exception):
@@ -103,6 +104,7 @@ _lib.oz2_profile_end.argtypes = [_vp, _P(StageTimes), _P(_i64)]
 _lib.oz2_unit_split.argtypes = [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp, _int, _int, _vp, _i64, _i64]
 _lib.oz2_unit_crt.argtypes = [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _int, _dbl, _dbl, _vp, _i64]
 _lib.oz2_unit_gemm.argtypes = [_vp, _vp, _vp, _i64, _i64, _i64, _int, _int, _vp, _i64]
+_lib.oz2_ozaki1_dgemm.argtypes = [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int]
 
 
 class GemmDesc(ctypes.Structure):
@@ -138,7 +140,7 @@ EXPORTS = [
     "oz2_precision_bits_complex", "oz2_moduli_for_bits", "oz2_set_workspace_limit", "oz2_set_resident_hint",
     "oz2_release_workspace", "oz2_reserve_workspace", "oz2_workspace_query", "oz2_stage_times", "oz2_last_tiling",
     "oz2_set_tuning", "oz2_get_tuning", "oz2_set_profiling", "oz2_profile_begin", "oz2_profile_end", "oz2_moduli",
-    "oz2_unit_split", "oz2_unit_crt", "oz2_unit_gemm", "oz2_batch_create", "oz2_batch_run", "oz2_batch_reselect", "oz2_batch_moduli", "oz2_batch_tiles",
+    "oz2_unit_split", "oz2_unit_crt", "oz2_unit_gemm", "oz2_ozaki1_dgemm", "oz2_batch_create", "oz2_batch_run", "oz2_batch_reselect", "oz2_batch_moduli", "oz2_batch_tiles",
     "oz2_batch_info", "oz2_batch_destroy", "oz2_batch_autotune", "oz2_selector_features", "oz2_selector_choose",
     "oz2_selector_set_weights", "oz2_selector_get_weights", "oz2_mlp_forward", "oz2_mlp_backward", "oz2_mlp_adam",
     "oz2_last_error", "oz2_version",
@@ -315,6 +317,10 @@ class Ozaki2:
         _check(_lib.oz2_dgemm_noalloc(self._h, _TRANS.get(ta, -1), _TRANS.get(tb, -1), m, n, k, float(alpha), _ptr(A), lda,
                                       _ptr(B), ldb, float(beta), _ptr(C), ldc, num_moduli, dep, len(deps), done))
 
+    def ozaki1(self, m, n, k, A, lda, B, ldb, C, ldc, L):
+        """Ozaki Scheme I with L slices (oz2_ozaki1_dgemm): C = A B, op N, alpha 1, beta 0"""
+        _check(_lib.oz2_ozaki1_dgemm(self._h, m, n, k, _ptr(A), lda, _ptr(B), ldb, _ptr(C), ldc, int(L)))
+
     def zgemm(self, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, num_moduli, deps=(), done=None):
         al = (_dbl * 2)(complex(alpha).real, complex(alpha).imag)
         be = (_dbl * 2)(complex(beta).real, complex(beta).imag)
@@ -371,15 +377,15 @@ class Ozaki2:
         _check(_lib.oz2_last_tiling(self._h, ctypes.byref(ti)))
         return ti
 
-    def set_tuning(self, batched_gemm=True, pad_lda=True, pow2_tiles=True, gemm_tile=-1):
-        t = Tuning(int(batched_gemm), int(pad_lda), int(pow2_tiles), int(gemm_tile))
+    def set_tuning(self, batched_gemm=True, pad_lda=True, pow2_tiles=True, gemm_tile=-1, overlap_blocks=-1):
+        t = Tuning(int(batched_gemm), int(pad_lda), int(pow2_tiles), int(gemm_tile), int(overlap_blocks))
         _check(_lib.oz2_set_tuning(self._h, ctypes.byref(t)))
 
     def tuning(self):
         t = Tuning()
         _check(_lib.oz2_get_tuning(self._h, ctypes.byref(t)))
         return {"batched_gemm": bool(t.batched_gemm), "pad_lda": bool(t.pad_lda), "pow2_tiles": bool(t.pow2_tiles),
-                "gemm_tile": t.gemm_tile}
+                "gemm_tile": t.gemm_tile, "overlap_blocks": t.overlap_blocks}
 
     def profile_begin(self):
         _check(_lib.oz2_profile_begin(self._h))

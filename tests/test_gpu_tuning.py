@@ -33,9 +33,13 @@ def tuned(oz, **kw):
 
 
 def test_roundtrip(oz):
-    assert oz.tuning() == {"batched_gemm": True, "pad_lda": True, "pow2_tiles": True, "gemm_tile": -1}
-    with tuned(oz, batched_gemm=False, pad_lda=False, pow2_tiles=False, gemm_tile=128):
-        assert oz.tuning() == {"batched_gemm": False, "pad_lda": False, "pow2_tiles": False, "gemm_tile": 128}
+    assert oz.tuning() == {"batched_gemm": True, "pad_lda": True, "pow2_tiles": True, "gemm_tile": -1,
+                           "overlap_blocks": -1}
+    with tuned(oz, batched_gemm=False, pad_lda=False, pow2_tiles=False, gemm_tile=128, overlap_blocks=3):
+        assert oz.tuning() == {"batched_gemm": False, "pad_lda": False, "pow2_tiles": False, "gemm_tile": 128,
+                               "overlap_blocks": 3}
+    with pytest.raises(oz2.OZ2InvalidArgument):
+        oz.set_tuning(overlap_blocks=17)
     assert oz.tuning()["batched_gemm"] is True
 
 
@@ -97,3 +101,62 @@ def test_no_pow2_tiles(oz):
             oz.set_workspace_limit(0)
     log(f"no-pow2 tiling {ti.as_dict()}")
     assert ti.pm * ti.pn > 1
+
+
+@pytest.mark.parametrize("blocks", [2, 3, 5])
+def test_column_block_overlap_matches_oracle(oz, blocks):
+    # column blocks of each panel pair over two streams (B split and CRT of one block beside the GEMM
+    # of another), on real and complex data, with several panels, k-chunks and per-plane launches
+    with tuned(oz, overlap_blocks=blocks):
+        real_case(oz, "N", "N", 700, 900, 500, 16, "phi", alpha=1.5, beta=0.5, check_bound=False)
+        real_case(oz, "T", "T", 300, 1000, 70000, 12, "u11", check_bound=False)
+        assert oz.last_tiling().k_chunks == 2
+        complex_case(oz, "C", "N", 260, 700, 300, 16, "phi", alpha=1 + 0.5j, beta=0.25 - 1j, check_bound=False)
+        oz.set_workspace_limit(8 << 20)
+        try:
+            real_case(oz, "N", "T", 1000, 1100, 400, 10, "phi", seed=5, check_bound=False)
+            ti = oz.last_tiling()
+        finally:
+            oz.set_workspace_limit(0)
+        assert ti.pm * ti.pn > 1
+    with tuned(oz, overlap_blocks=blocks, batched_gemm=False):
+        real_case(oz, "N", "T", 500, 777, 300, 9, "phi", check_bound=False)
+
+
+def test_overlap_stage_times_and_graph_capture(oz):
+    # stage times come from both streams; a capture of the caller's stream takes the second stream's
+    # work into the graph (event fork and join), replayed several times with eager work in between
+    import torch
+    from gpu_cases import to_dev
+    from oracle import oracle as O
+    rng = np.random.default_rng(3)
+    m, n, k, s = 1024, 2048, 1024, 16
+    A, B = rng.uniform(-1, 1, m * k), rng.uniform(-1, 1, k * n)
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(np.zeros(m * n))
+    want = np.zeros(m * n)
+    O.dgemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, want, m, s)
+    st = torch.cuda.Stream()
+    h = oz2.Ozaki2(st)
+    try:
+        h.set_tuning(overlap_blocks=4)
+        with torch.cuda.stream(st):
+            h.gemm("N", "N", m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, s)
+        st.synchronize()
+        t = h.stage_times()
+        assert min(t.scale_ms, t.split_ms, t.gemm_ms, t.crt_ms) > 0
+        assert np.array_equal(bits(dC.cpu().numpy()), bits(want))
+        # during capture: no stage events, no device-memory queries (a fixed limit), no growth
+        h.set_workspace_limit(4 << 30)
+        h.reserve_workspace(h.workspace_query("N", "N", m, n, k, s))
+        h.set_profiling(False)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=st):
+            h.gemm("N", "N", m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, s)
+        for rep in range(3):
+            dC.zero_()
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert np.array_equal(bits(dC.cpu().numpy()), bits(want)), f"replay {rep}"
+    finally:
+        h.close()

@@ -1,4 +1,4 @@
-"""Host-side pieces of the Ozaki Scheme I harness (oz2/ozaki1.py) on CPU tensors: the
+"""Host-side pieces of the torch restatement of Ozaki Scheme I (tests/ozaki1_torch.py) on CPU tensors: the
 exact power-of-two scaling against math.ldexp (one correctly rounded step, including
 subnormal results, underflow to zero and overflow), and the whole scheme on CPU torch
 bit-identical to the exact oracle (oracle/ozaki1.py)."""
@@ -10,7 +10,7 @@ import torch
 
 from conftest import make_mat
 from oracle import ozaki1 as O1
-from oz2 import ozaki1
+import ozaki1_torch as ozaki1
 
 
 def test_ldexp_matches_math(rng):
