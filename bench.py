@@ -29,9 +29,19 @@ INT8_PEAK_TOPS = 5000.0  # MI355X dense INT8 MFMA, 5.0 POP/s
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E, 8 TB/s
 
 
-# HBM bytes per INT8 GEMM launch measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE passes,
-# each scaled by its reading for a 1 GiB copy kernel): profiles/dgemm8192_s16_hbm_traffic.txt
-MEASURED_GEMM_TRAFFIC = {(8192, 8192, 8192, 16): 14.71e9}
+# HBM bytes per INT8 GEMM launch measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE passes of this
+# build's bench.py workloads on one MI355X, each scaled by its reading for a 1 GiB copy kernel).
+# A bench run cannot count its own traffic (the counters need rocprofv3 around the process), so the
+# line quotes these with their source; the GEMM's fetch (its L2 misses) differs by a few % between boxes.
+MEASURED_GEMM_TRAFFIC = {
+    ("dgemm", 8192, 8192, 8192, 16): (1.4467e10, "profiles/dgemm8192_s16_hbm_traffic.txt"),
+    ("zgemm", 4096, 4096, 4096, 16): (5.6766e9, "profiles/zgemm4096_s16_hbm_traffic.txt"),
+}
+
+
+def gemm_traffic(kind, m, n, k, s):
+    t = MEASURED_GEMM_TRAFFIC.get((kind, m, n, k, s))
+    return (t[0], t[1]) if t else (None, None)
 
 
 def parse():
@@ -257,7 +267,10 @@ def zgemm_line(args, torch):
                                   "lda=k+7 ldc=m+3", "tk": oz2.Ozaki2.precision_bits_complex(k, s),
                       "tiling": oz.last_tiling().as_dict()},
            "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
-                        "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
+                        "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": gemm_traffic("zgemm", m, n, k, s)[0],
+                        "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                        "traffic_source": gemm_traffic("zgemm", m, n, k, s)[1],
+                        "algorithmic_bytes": 3 * s * (k * (m + n) + m * n),
                         "kernel": "igemm_residue_kernel (2*m*n*k*3s int8 ops per launch)", "gemm_ms": round(gemm_ms, 4)},
            "stage_ms": {kk: round(v / max(1, calls), 4) for kk, v in st.as_dict().items()}}
     out["hbm_stages"] = hbm_stages(m, n, k, s, True, True, out["stage_ms"])
@@ -609,8 +622,10 @@ def main():
         "config": {"workload": f"DGEMM {m}x{n}x{k} op N/N s={s} alpha=1.5 beta=0.5, A B C0 U[-1,1)", "m": m, "n": n,
                    "k": k, "s": s, "tk": oz2.Ozaki2.precision_bits(k, s), "tiling": tiling},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": MEASURED_GEMM_TRAFFIC.get((m, n, k, s)),
+                     "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": gemm_traffic("dgemm", m, n, k, s)[0],
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "traffic_source": gemm_traffic("dgemm", m, n, k, s)[1],
+                     "algorithmic_bytes": s * k * (m + n) + s * m * n,
                      "kernel": "igemm_residue_kernel (2*m*n*k*s int8 ops per launch)",
                      "gemm_ms": round(gemm_ms, 4)},
         "stage_ms": {kk: round(v, 4) for kk, v in stage.items()},

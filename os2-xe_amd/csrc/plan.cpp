@@ -102,16 +102,31 @@ Plan make_plan(int s, int64_t k) {
     p.NL = std::max(2, (p.M.bits() - 1) / 32 + 1);
     if (p.NL != limbs_for(s)) throw std::logic_error("oz2: limb count mismatch with the moduli table");
 
-    // tk: the largest t <= kMaxScaleBits with k * 2^(2t+2) <= M.  Since the left side is an
-    // integer, k * 2^(2t+2) <= M  <=>  2^(2t+2) <= floor(M / k)  <=>  2t + 2 <= bits(floor(M / k)) - 1,
-    // so tk = (bits(floor(M / k)) - 3) / 2 rounded down, exactly, from one division.
-    Wide q = p.M;
-    q.divmod(static_cast<uint64_t>(k));
-    if (q.bits() < 5)  // floor(M / k) < 16: not even t = 1 fits
+    p.tk = scale_bits(s, k);
+    if (p.tk < 1)
         throw std::invalid_argument("oz2: num_moduli=" + std::to_string(s) + " is too small for k=" +
                                     std::to_string(k));
-    p.tk = std::min((q.bits() - 3) / 2, kMaxScaleBits);
     return p;
+}
+
+int scale_bits(int s, int64_t k) {
+    // M per s, built once
+    static const struct Products {
+        Wide M[kMaxModuli + 1];
+        Products() {
+            for (int t = kMinModuli; t <= kMaxModuli; ++t) {
+                M[t] = Wide::of(1);
+                for (int u = 0; u < t; ++u) M[t].mul(moduli(t)[u]);
+            }
+        }
+    } prod;
+    // Since k 2^(2t+2) is an integer, k 2^(2t+2) <= M  <=>  2^(2t+2) <= floor(M / k)  <=>
+    // 2t + 2 <= bits(floor(M / k)) - 1, so tk = (bits(floor(M / k)) - 3) / 2 rounded down,
+    // exactly, from one division; floor(M / k) < 16 admits no t >= 1
+    Wide q = prod.M[s];
+    q.divmod(static_cast<uint64_t>(k));
+    if (q.bits() < 5) return 0;
+    return std::min((q.bits() - 3) / 2, kMaxScaleBits);
 }
 
 // ---- tables -------------------------------------------------------------------
@@ -155,12 +170,9 @@ void make_tabsets(TabSet* out) {
 }
 
 int moduli_for_bits(int64_t k, int bits) {
-    for (int s = kMinModuli; s <= kMaxModuli; ++s) {
-        try {
-            if (make_plan(s, k).tk >= bits) return s;
-        } catch (const std::invalid_argument&) {
-        }
-    }
+    if (k < 1) return kMaxModuli;  // no plan admits k < 1 (hpp:693-700 then ends at the largest set)
+    for (int s = kMinModuli; s <= kMaxModuli; ++s)
+        if (scale_bits(s, k) >= bits) return s;
     return kMaxModuli;
 }
 

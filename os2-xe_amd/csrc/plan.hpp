@@ -67,6 +67,9 @@ struct Plan {
 // Throws std::invalid_argument (s out of range, k < 1, s too small for k),
 // std::runtime_error (bad table) or std::logic_error (limb count).
 Plan make_plan(int s, int64_t k);
+// make_plan(s, k).tk without building the plan or throwing: the largest t <= kMaxScaleBits with
+// k 2^(2t+2) <= M, or 0 when s moduli do not admit k (s in [kMinModuli, kMaxModuli], k >= 1)
+int scale_bits(int s, int64_t k);
 // smallest s whose plan keeps >= bits per element; kMaxModuli if none does (hpp:693-700)
 int moduli_for_bits(int64_t k, int bits);
 

@@ -108,11 +108,7 @@ std::vector<int> Selector::choose(hipStream_t st, const std::vector<SelProblem>&
         int pick = 0, largest = 0;
         for (int t = 0; t < kSelOutputs; ++t) {
             const int si = t + kMinModuli;
-            try {
-                (void)make_plan(si, k);  // s must admit this k
-            } catch (const std::invalid_argument&) {
-                continue;
-            }
+            if (scale_bits(si, k) < 1) continue;  // s must admit this k
             largest = si;
             if (!pick && pr[i * kSelOutputs + t] >= tau_) pick = si;
         }
