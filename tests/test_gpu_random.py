@@ -2,7 +2,8 @@
 shapes (m, n < 320; k up to 3000), every op(A)/op(B), every admissible s, random
 alpha/beta, padded leading dimensions, and random tuning (per-plane launches,
 plane padding, power-of-two candidates, forced GEMM tiles) and small workspace
-limits that force multi-tile runs.  Every C must be bit-identical to the oracle on
+limits that force multi-tile runs, column blocks overlapped over two streams.  Every C
+must be bit-identical to the oracle on
 the same inputs (the reference's algorithm restated; tests/test_ozaki2.cpp:95-194
 runs one fixed case per behaviour, this covers their combinations).  Random batches
 (oz2_batch_*) of such problems, with random tiles, schedules, order and graph use,
@@ -34,7 +35,8 @@ def test_random_cases_match_oracle(seed):
                 s = int(rng.integers(3, 25))
             dist = str(rng.choice(["uniform", "phi", "ints"]))
             tune = dict(batched_gemm=bool(rng.random() < 0.8), pad_lda=bool(rng.random() < 0.5),
-                        pow2_tiles=bool(rng.random() < 0.5), gemm_tile=int(rng.choice([-1, -1, -1, 256, 128, 64, 256128])))
+                        pow2_tiles=bool(rng.random() < 0.5), gemm_tile=int(rng.choice([-1, -1, -1, 256, 128, 64, 256128])),
+                        overlap_blocks=int(rng.choice([-1, -1, 2, 3])))
             limit = int(rng.choice([4, 8, 16])) << 20 if rng.random() < 0.15 else 0
             oz.set_tuning(**tune)
             oz.set_workspace_limit(limit)
@@ -97,3 +99,31 @@ def test_random_batches_match_single_calls(seed):
             b.close()
     finally:
         oz.close()
+
+
+@pytest.mark.parametrize("seed", [606])
+def test_random_noalloc_matches_oracle(seed):
+    # the allocation-free entry point on random shapes, ops, s and alpha/beta in a row (its scratch
+    # keeps the previous call's contents), against the oracle
+    rng = np.random.default_rng(seed)
+    with oz2.Ozaki2() as oz:
+        for c in range(20):
+            m, n = (int(x) for x in rng.integers(1, 700, 2))
+            k = int(rng.integers(1, 5000))
+            ta, tb = str(rng.choice(["N", "T"])), str(rng.choice(["N", "T"]))
+            s = int(rng.integers(3, 25))
+            while O.precision_bits(k, s) < 0:
+                s = int(rng.integers(3, 25))
+            alpha = float(rng.uniform(-2, 2)) if rng.random() < 0.5 else 1.0
+            beta = float(rng.uniform(-2, 2)) if rng.random() < 0.5 else 0.0
+            ar, ac = (m, k) if ta == "N" else (k, m)
+            br, bc = (k, n) if tb == "N" else (n, k)
+            A, B = rng.uniform(-1, 1, ar * ac), rng.uniform(-1, 1, br * bc)
+            C0 = rng.uniform(-1, 1, m * n)
+            log(f"random noalloc {seed}/{c}: {ta}{tb} m={m} n={n} k={k} s={s} alpha={alpha} beta={beta}")
+            dC = to_dev(C0.copy())
+            oz.gemm_noalloc(ta, tb, m, n, k, alpha, to_dev(A), ar, to_dev(B), br, beta, dC, m, s)
+            torch.cuda.synchronize()
+            want = C0.copy()
+            O.dgemm(ta, tb, m, n, k, alpha, A, ar, B, br, beta, want, m, s)
+            assert np.array_equal(dC.cpu().numpy().view(np.uint64), want.view(np.uint64))
