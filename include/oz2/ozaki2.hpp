@@ -199,6 +199,13 @@ public:
     Batch(const Batch&) = delete;
     Batch& operator=(const Batch&) = delete;
     void run() { detail::check(oz2_batch_run(b_)); }
+    // choose the moduli again from the buffers' current contents (oz2_batch_reselect); returns how
+    // many problems changed
+    int reselect() {
+        int changed = 0;
+        detail::check(oz2_batch_reselect(b_, &changed));
+        return changed;
+    }
     std::vector<int> moduli() const {
         std::vector<int> s(n_);
         detail::check(oz2_batch_moduli(b_, s.data()));
