@@ -77,6 +77,7 @@ __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __rest
     };
     __shared__ unsigned int smax, sgmax;
     __shared__ unsigned long long scnt[kBins + 2];
+    __shared__ uint64_t vals[kSelCols];  // the sampled magnitudes: read from memory once
     if (threadIdx.x == 0) {
         smax = 0;
         sgmax = 0;
@@ -84,7 +85,11 @@ __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __rest
     if (threadIdx.x < kBins + 2) scnt[threadIdx.x] = 0;
     __syncthreads();
     unsigned int emax = 0;
-    for (int j = threadIdx.x; j < np; j += 256) emax = max(emax, static_cast<unsigned int>(at(j) >> 52));
+    for (int j = threadIdx.x; j < np; j += 256) {
+        const uint64_t b = at(j);
+        vals[j] = b;
+        emax = max(emax, static_cast<unsigned int>(b >> 52));
+    }
     atomicMax(&smax, emax);
     __syncthreads();
     const unsigned int e0 = smax;
@@ -94,7 +99,7 @@ __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __rest
 #pragma unroll
     for (int b = 0; b <= kBins; ++b) lc[b] = 0;
     for (int j = threadIdx.x; j < np; j += 256) {
-        const uint64_t bits = at(j);
+        const uint64_t bits = vals[j];
         if (bits == 0) {
             ++lc[kBins];
             continue;
