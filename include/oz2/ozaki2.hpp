@@ -217,4 +217,20 @@ private:
     int n_ = 0;
 };
 
+// Ozaki Scheme I, the reference's comparison baseline (oz1::Ozaki1, benchmarks/ozaki1_gpu.hpp:86-155):
+// C = A B (op N, alpha 1, beta 0) with L slices, on an Ozaki2 handle's stream and workspace.
+class Ozaki1 {
+public:
+    explicit Ozaki1(Ozaki2& oz) : oz_(oz) {}
+    static int precision_bits(int L) { return 7 * L - 2; }
+    static int num_gemms(int L) { return L * (L + 1) / 2; }
+    void gemm(int64_t m, int64_t n, int64_t k, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+              int64_t ldc, int L) {
+        detail::check(oz2_ozaki1_dgemm(oz_.native_handle(), m, n, k, A, lda, B, ldb, C, ldc, L));
+    }
+
+private:
+    Ozaki2& oz_;
+};
+
 }  // namespace oz2

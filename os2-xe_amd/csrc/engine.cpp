@@ -248,7 +248,15 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
                  const double* B, int64_t ldb, double br, double bi, double* C, int64_t ldc, int s, bool cplx,
                  std::byte* fixed_ws, size_t fixed_bytes) {
     // a call into a fixed workspace creates nothing: no stage events either
-    const bool timed = profiling_ && !fixed_ws;
+    // under stream capture: no timing events, and the panels are planned for the workspace as it
+    // is (it cannot grow there: that needs a synchronization and an allocation)
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream_, &cap) != hipSuccess) {
+        (void)hipGetLastError();
+        cap = hipStreamCaptureStatusNone;
+    }
+    const bool capturing = cap == hipStreamCaptureStatusActive;
+    const bool timed = profiling_ && !fixed_ws && !capturing;
     if (!accumulate_) marks_.clear();
     sync_used_ = 0;
     ++calls_;
@@ -277,7 +285,8 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
                                         std::to_string((need + (size_t{1} << 20) - 1) >> 20) + " MiB, it holds " +
                                         std::to_string(fixed_bytes >> 20) + " MiB");
     }
-    const TileInfo ti = plan_for_call(cs, fixed_ws ? fixed_bytes : budget());
+    const size_t limit = fixed_ws ? fixed_bytes : capturing && ws_limit_ == 0 ? ws_size_ : budget();
+    const TileInfo ti = plan_for_call(cs, limit);
     const int64_t mb = ti.mb, nb = ti.nb;
     const Layout L = layout(cs, mb, nb);
     const int64_t kp = L.kp, kch = L.kch, klen = L.klen;

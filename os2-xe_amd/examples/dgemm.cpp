@@ -53,6 +53,13 @@ int main(int argc, char** argv) {
     batch.run();
     if (hipStreamSynchronize(stream)) return 1;
     std::printf("batch: learned s = %d (moduli_for_bits: %d)\n", batch.moduli()[0], s);
+
+    // the reference's comparison baseline, Ozaki Scheme I with 9 slices (45 INT8 GEMMs, 61 bits)
+    oz2::Ozaki1 oz1(oz);
+    oz1.gemm(m, n, k, dA, m, dB, k, dC, m, 9);
+    if (hipStreamSynchronize(stream)) return 1;
+    std::printf("ozaki1: L=9, %d INT8 GEMMs, %d bits per element\n", oz2::Ozaki1::num_gemms(9),
+                oz2::Ozaki1::precision_bits(9));
     (void)hipFree(dA);
     (void)hipFree(dB);
     (void)hipFree(dC);

@@ -160,3 +160,38 @@ def test_overlap_stage_times_and_graph_capture(oz):
             assert np.array_equal(bits(dC.cpu().numpy()), bits(want)), f"replay {rep}"
     finally:
         h.close()
+
+
+def test_capture_with_default_settings():
+    # a capture of a plain call with the default settings (stage timing on, no workspace limit): the
+    # library records no timing events and plans for the workspace reserved before the capture;
+    # every replay, with eager work in between, equals the eager result
+    import torch
+    from gpu_cases import to_dev
+    rng = np.random.default_rng(8)
+    m, n, k, s = 700, 900, 1500, 16
+    A, B = rng.uniform(-1, 1, m * k), rng.uniform(-1, 1, k * n)
+    C0 = rng.uniform(-1, 1, m * n)
+    dA, dB = to_dev(A), to_dev(B)
+    st = torch.cuda.Stream()
+    h = oz2.Ozaki2(st)
+    try:
+        eager = to_dev(C0.copy())
+        with torch.cuda.stream(st):
+            h.gemm("N", "N", m, n, k, 1.5, dA, m, dB, k, 0.5, eager, m, s)
+        st.synchronize()
+        want = eager.cpu().numpy()
+        h.reserve_workspace(h.workspace_query("N", "N", m, n, k, s))
+        dC = to_dev(C0.copy())
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=st):
+            h.gemm("N", "N", m, n, k, 1.5, dA, m, dB, k, 0.5, dC, m, s)
+        for rep in range(3):
+            dC.copy_(to_dev(C0))
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert np.array_equal(bits(dC.cpu().numpy()), bits(want)), f"replay {rep}"
+    finally:
+        h.close()
