@@ -131,7 +131,7 @@ BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector
                 a.s = s_[i];
                 a.cplx = false;
                 a.conj = false;
-                a.wide = tk > kNarrowScaleBits;
+                a.form = limb_form(tk);
                 a.tab = tabs + s_[i];
                 pj.push_back(a);
             }

@@ -486,7 +486,7 @@ int oz2_unit_split(void* stream, const double* X, int64_t ldx, int64_t rows, int
         a.kcontig = kcontig != 0;
         a.cplx = cplx != 0;
         a.conj = conj != 0;
-        a.wide = tk > oz2::kNarrowScaleBits;
+        a.form = oz2::limb_form(tk);
         a.tab = unit_tables() + num_moduli;
         oz2::k::split(static_cast<hipStream_t>(stream), a);
         if (hipGetLastError() != hipSuccess) throw std::runtime_error("oz2: split launch failed");

@@ -273,7 +273,7 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
     }
     const Plan& p = plan_for(s, cplx ? 2 * k : k);  // complex plan at 2k (hpp:778-781)
     const int tk = p.tk;
-    const bool wide = tk > kNarrowScaleBits;
+    const int form = limb_form(tk);
     const TabSet* tab = dtabs_ + s;
 
     const CallShape cs = call_shape(ta, tb, m, n, k, s, cplx);
@@ -334,7 +334,7 @@ void Engine::run(Op ta, Op tb, int64_t m, int64_t n, int64_t k, double ar, doubl
         sp.s = s;
         sp.cplx = cplx;
         sp.conj = cplx && op == Op::C;
-        sp.wide = wide;
+        sp.form = form;
         sp.tab = tab;
         k::split(on, sp);
     };

@@ -59,7 +59,8 @@ struct SplitArgs {
     int64_t plane;        // bytes per plane (rows_pad * kp)
     int64_t group;        // bytes per group of s planes (complex)
     int s;
-    bool kcontig, cplx, conj, wide;
+    bool kcontig, cplx, conj;
+    int form;             // limb_form(tk): octets of the offset value 8, 12 or 14
     const TabSet* tab;    // device tables of this s
 };
 void split(hipStream_t st, const SplitArgs& a);

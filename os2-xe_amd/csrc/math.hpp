@@ -136,26 +136,30 @@ OZ2_HD uint64_t scaled_mag62(uint64_t bits, int e) {
     return q + (rem + (q & 1) > half ? 1 : 0);
 }
 
-// Octet forms of the offset-binary value u = 2^OFF + x (x = +-RNE(|v| 2^e)), kept
-// as little-endian 32-bit words:
-//   Octets8  OFF = 62 (tk <= 62): the 8 bytes of u <= 2^63 in a[0..1];
-//   Octets14 OFF = 104 (tk > 62): the 14 bytes of u <= 2^105 in a[0..3].
-enum class Limbs { Octets8, Octets14 };
+// Octet forms of the offset-binary value u = 2^OFF + x (x = +-RNE(|v| 2^e), |x| <= 2^tk),
+// kept as little-endian 32-bit words; the residue chain takes one v_dot4 per word:
+//   Octets8  OFF = 62  (tk <= 62): the 8 bytes of u <= 2^63 in a[0..1];
+//   Octets12 OFF = 95  (tk <= 94): the 12 bytes of u < 2^96 in a[0..2] (every plan: tk <= 91);
+//   Octets14 OFF = 104 (tk <= 100): the 14 bytes of u <= 2^105 in a[0..3].
+enum class Limbs { Octets8, Octets12, Octets14 };
+template <Limbs F>
+inline constexpr int kLimbWords = F == Limbs::Octets8 ? 2 : F == Limbs::Octets12 ? 3 : 4;
 
 // zero: the element is outside the matrix or its row is non-finite, giving x = 0.
 template <Limbs F>
 OZ2_HD void to_limbs(uint64_t bits, int e, bool zero, uint32_t (&a)[4]) {
     const bool neg = (bits >> 63) != 0 && !zero;
-    if (F == Limbs::Octets14) {  // mag <= 2^100
+    if (F != Limbs::Octets8) {  // mag <= 2^100; the high word of u is 2^(OFF - 64) +- hi
+        constexpr int kTop = F == Limbs::Octets12 ? 31 : 40;
         uint64_t hi = 0, lo = 0;
         if (!zero) scaled_mag(bits, e, hi, lo);
         uint64_t uh, ul;
         if (neg) {
             ul = 0 - lo;
-            uh = (1ull << 40) - hi - (lo != 0 ? 1 : 0);
+            uh = (1ull << kTop) - hi - (lo != 0 ? 1 : 0);
         } else {
             ul = lo;
-            uh = (1ull << 40) + hi;
+            uh = (1ull << kTop) + hi;
         }
         a[0] = static_cast<uint32_t>(ul);
         a[1] = static_cast<uint32_t>(ul >> 32);
@@ -189,18 +193,16 @@ OZ2_HD uint32_t sym_m23(uint32_t w, uint32_t m, uint32_t mu, uint32_t h2) {
 
 // Symmetric residue r - h of x for modulus index t >= 1 (m <= 255), r = (x + h) mod m,
 // as a 32-bit value whose low byte is the int8 plane entry, from the octets of
-// u = 2^OFF + x: v = (h - 2^OFF mod m) + sum_j byte_j(u) (2^(8 j) mod m) <= 14 * 255 * 254
+// u = 2^OFF + x: v = (h - 2^OFF mod m) + sum_j byte_j(u) (2^(8 j) mod m) <= 16 * 255 * 254
 // + 254 < 2^20, one v_dot4 per word.  The chain starts at off - h (mod 2^32), so it
 // yields w = v - h for sym_m23.
 template <Limbs F, typename Tab>
 OZ2_HD uint32_t sym_residue(const uint32_t (&a)[4], Tab tb, int t) {
-    constexpr bool W = F == Limbs::Octets14;
-    uint32_t w = udot4(a[0], tb->c8[t][0], tb->offh[W ? 1 : 0][t]);
+    constexpr int f = static_cast<int>(F);
+    uint32_t w = udot4(a[0], tb->c8[t][0], tb->offh[f][t]);
     w = udot4(a[1], tb->c8[t][1], w);
-    if (W) {
-        w = udot4(a[2], tb->c8[t][2], w);
-        w = udot4(a[3], tb->c8[t][3], w);
-    }
+    if (kLimbWords<F> > 2) w = udot4(a[2], tb->c8[t][2], w);
+    if (kLimbWords<F> > 3) w = udot4(a[3], tb->c8[t][3], w);
     return sym_m23(w, tb->m[t], tb->mu[t], tb->h2[t]);
 }
 

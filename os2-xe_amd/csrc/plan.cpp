@@ -152,10 +152,11 @@ void make_tabsets(TabSet* out) {
             tb.mu[t] = static_cast<uint32_t>(((uint64_t{1} << 31) + m - 1) / m);
             tb.h[t] = h;
             tb.c16[t] = pow2_mod(16, m);
-            tb.off[0][t] = (m - pow2_mod(62, m) + h) % m;
-            tb.off[1][t] = (m - pow2_mod(104, m) + h) % m;
-            tb.offh[0][t] = tb.off[0][t] - h;
-            tb.offh[1][t] = tb.off[1][t] - h;
+            const int offs[3] = {62, 95, 104};
+            for (int f = 0; f < 3; ++f) {
+                tb.off[f][t] = (m - pow2_mod(offs[f], m) + h) % m;
+                tb.offh[f][t] = tb.off[f][t] - h;
+            }
             tb.h2[t] = 2 * h;
             for (int j = 0; j < 16; ++j) tb.c8[t][j >> 2] |= static_cast<uint32_t>(pow2_mod(8 * j, m)) << (8 * (j & 3));
             tb.c30[t] = m == 256 ? 0 : m * static_cast<uint32_t>(((uint64_t{1} << 30) + m - 1) / m);

@@ -16,6 +16,9 @@ inline constexpr int kMinModuli = 2;
 inline constexpr int kMaxLimbs = 6;           // 32-bit words of M for s = 24
 inline constexpr int kMaxScaleBits = 100;     // tk cap (hpp:29)
 inline constexpr int kNarrowScaleBits = 62;   // tk <= 62: |x| < 2^63 (hpp:30)
+inline constexpr int kWideScaleBits = 94;     // tk <= 94: 2^95 + x fits 96 bits (every plan has tk <= 91)
+// octet form of the split for a scaling width tk: 0 (8 octets), 1 (12), 2 (14)
+constexpr int limb_form(int tk) { return tk <= kNarrowScaleBits ? 0 : tk <= kWideScaleBits ? 1 : 2; }
 inline constexpr int32_t kNonFinite = INT32_MIN;
 // k-chunks of the INT8 GEMM.  The reference splits k at 131071 (hpp:32), the
 // largest chunk whose int32 accumulation of int8 products is exact
@@ -85,8 +88,8 @@ struct TabSet {
     uint32_t mu[kMaxModuli];                // ceil(2^31 / m): exact quotients of values below 2^23 (mod_m23)
     uint32_t h[kMaxModuli];                 // (m - 1) / 2
     uint32_t c16[kMaxModuli];               // 2^16 mod m
-    uint32_t off[2][kMaxModuli];            // (m - 2^OFF mod m + h) mod m, OFF = 62 (narrow), 104 (wide)
-    uint32_t offh[2][kMaxModuli];           // off - h (mod 2^32): start of the symmetric split's dot chain
+    uint32_t off[3][kMaxModuli];            // (m - 2^OFF mod m + h) mod m, OFF = 62, 95, 104 (limb_form 0, 1, 2)
+    uint32_t offh[3][kMaxModuli];           // off - h (mod 2^32): start of the symmetric split's dot chain
     uint32_t h2[kMaxModuli];                // 2 h
     uint32_t c8[kMaxModuli][4];             // bytes 2^(8 j) mod m, j = 0..15, four to a word (low first)
     uint32_t c30[kMaxModuli];               // m * ceil(2^30 / m) (GEMM residue epilogue offset; 0 for m = 256)

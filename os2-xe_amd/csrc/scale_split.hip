@@ -13,6 +13,7 @@
 // that are strided in memory (op(A) = N, op(B) = T) are read column-by-column
 // with lanes along rows, so no separate transpose pass is needed.
 #include <algorithm>
+#include <stdexcept>
 
 #include "grouped.hpp"
 #include "kernels.hpp"
@@ -238,7 +239,7 @@ __device__ __forceinline__ void store_bytes(int8_t* p, const uint32_t (&w)[4]) {
 // Q-byte store per plane.  All loads are issued before any arithmetic and stay
 // inside the matrix: padding rows read row 0 and k past the end reads k - 1, and
 // those values are then zeroed.
-template <bool WIDE, bool KCONTIG, int Q>
+template <int FORM, bool KCONTIG, int Q>
 __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, uint32_t bx, uint32_t by) {
     const auto tb = const_tab(a.tab);
     int64_t r, ck;
@@ -273,8 +274,8 @@ __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, 
         }
     }
 
-    // narrow values: the 8 bytes of the offset value, wide ones: its 14 bytes
-    constexpr Limbs F = WIDE ? Limbs::Octets14 : Limbs::Octets8;
+    // the 8, 12 or 14 bytes of the offset value (limb_form of tk)
+    constexpr Limbs F = static_cast<Limbs>(FORM);
     uint32_t limbs[Q][4];
 #pragma unroll
     for (int q = 0; q < Q; ++q) to_limbs<F>(rowzero ? 0 : bits[q], e, false, limbs[q]);
@@ -297,7 +298,7 @@ __device__ __forceinline__ void split_real_body(const SplitArgs& a, int lg_tpr, 
 // Complex split: three plane groups Re, Im (negated for conj), Re+Im, each
 // reduced to the symmetric range [-h, m-1-h]; slots, k order and loads as in the
 // real split, one Q-byte store per plane of each group.
-template <bool WIDE, bool KCONTIG, bool CONJ, int Q>
+template <int FORM, bool KCONTIG, bool CONJ, int Q>
 __device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, uint32_t bx, uint32_t by) {
     const auto tb = const_tab(a.tab);
     int64_t r, ck;
@@ -331,7 +332,7 @@ __device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, 
             bi[q] = p < a.k ? bi[q] : 0;
         }
     }
-    constexpr Limbs F = WIDE ? Limbs::Octets14 : Limbs::Octets8;
+    constexpr Limbs F = static_cast<Limbs>(FORM);
     uint32_t lr[Q][4], li[Q][4];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -368,26 +369,27 @@ __device__ __forceinline__ void split_cplx_body(const SplitArgs& a, int lg_tpr, 
     }
 }
 
-// split variants: bit 0 wide, 1 kcontig, 2 complex, 3 conj
+// split variants: bits 0-1 limb form (0, 1, 2), bit 2 kcontig, 3 complex, 4 conj
 struct SplitItem {
     SplitArgs a;
     int lg;
     uint32_t gx;
 };
-// bytes per thread slot: half chunks for wide complex values (their 2 x 16 octet words
+// bytes per thread slot: half chunks for wide complex values (their 2 x 4 octet words
 // per element would not fit 16 elements in registers), whole chunks otherwise
-constexpr int split_q(bool wide, bool cplx) { return wide && cplx ? 8 : 16; }
+constexpr int split_q(int form, bool cplx) { return form != 0 && cplx ? 8 : 16; }
 template <int V>
 __device__ __forceinline__ void split_body(const SplitItem& it, uint32_t bx, uint32_t by) {
-    constexpr bool W = V & 1, KC = (V >> 1) & 1, CX = (V >> 2) & 1, CJ = (V >> 3) & 1;
-    if constexpr (CX) split_cplx_body<W, KC, CJ, split_q(W, true)>(it.a, it.lg, bx, by);
-    else split_real_body<W, KC, split_q(W, false)>(it.a, it.lg, bx, by);
+    constexpr int FM = V & 3;
+    constexpr bool KC = (V >> 2) & 1, CX = (V >> 3) & 1, CJ = (V >> 4) & 1;
+    if constexpr (CX) split_cplx_body<FM, KC, CJ, split_q(FM, true)>(it.a, it.lg, bx, by);
+    else split_real_body<FM, KC, split_q(FM, false)>(it.a, it.lg, bx, by);
 }
 template <int V>
 __global__ void __launch_bounds__(256) split_kernel(SplitItem it) { split_body<V>(it, blockIdx.x, blockIdx.y); }
 
 int split_variant(const SplitArgs& a) {
-    return (a.wide ? 1 : 0) | (a.kcontig ? 2 : 0) | (a.cplx ? 4 : 0) | (a.cplx && a.conj ? 8 : 0);
+    return a.form | (a.kcontig ? 4 : 0) | (a.cplx ? 8 : 0) | (a.cplx && a.conj ? 16 : 0);
 }
 
 // grid of one split: KCONTIG: rows x slot groups (tpr threads per row); strided: 16 rows x 16 slots.
@@ -397,7 +399,7 @@ int split_variant(const SplitArgs& a) {
 // rows sharing a line came from different workgroups, usually on different XCDs, and
 // the L2s wrote half lines: 0.79 against 0.69 ms for both operands at 8192^3 s = 16.
 SplitItem split_item(const SplitArgs& a, uint32_t& gy) {
-    const int q = split_q(a.wide, a.cplx);
+    const int q = split_q(a.form, a.cplx);
     const int64_t nck = a.kp / q;
     SplitItem it{a, 0, 0};
     if (a.kcontig) {
@@ -472,10 +474,11 @@ void split(hipStream_t st, const SplitArgs& a) {
     const SplitItem it = split_item(a, gy);
     switch (split_variant(a)) {
 #define OZ2_V(V) case V: launch_split_v<V>(st, it, gy); break;
-        OZ2_V(0) OZ2_V(1) OZ2_V(2) OZ2_V(3) OZ2_V(4) OZ2_V(5) OZ2_V(6) OZ2_V(7)
-        OZ2_V(12) OZ2_V(13) OZ2_V(14) OZ2_V(15)
+#define OZ2_F(F) OZ2_V(F) OZ2_V(F + 4) OZ2_V(F + 8) OZ2_V(F + 12) OZ2_V(F + 24) OZ2_V(F + 28)
+        OZ2_F(0) OZ2_F(1) OZ2_F(2)
+#undef OZ2_F
 #undef OZ2_V
-        default: break;
+        default: throw std::logic_error("oz2: no split kernel for this variant");
     }
 }
 
@@ -506,7 +509,7 @@ void prep_scale(std::vector<GroupedLaunch>& out, const std::vector<ScaleJob>& jo
 }
 
 void prep_split(std::vector<GroupedLaunch>& out, const std::vector<SplitArgs>& jobs) {
-    GroupedLaunch by[16];
+    GroupedLaunch by[32];
     for (const SplitArgs& a : jobs) {
         if (a.rows_pad <= 0) continue;
         uint32_t gy = 0;
@@ -540,8 +543,8 @@ bool launch_scale_split_grouped(hipStream_t st, const GroupedLaunch& g) {
         hipLaunchKernelGGL((grouped_split_kernel<V>), grid, block, 0, st,                         \
                            static_cast<const SplitItem*>(g.d_items), g.d_start, g.n);             \
         return true;
-        OZ2_V(0) OZ2_V(1) OZ2_V(2) OZ2_V(3) OZ2_V(4) OZ2_V(5) OZ2_V(6) OZ2_V(7)
-        OZ2_V(12) OZ2_V(13) OZ2_V(14) OZ2_V(15)
+        // the batch takes real products only, whose plans all have tk <= 91 (forms 0 and 1)
+        OZ2_V(0) OZ2_V(1) OZ2_V(4) OZ2_V(5)
 #undef OZ2_V
         default: return false;
     }

@@ -27,6 +27,7 @@ Selector::Selector() {
 Selector::~Selector() {
     if (dw_) (void)hipFree(dw_);
     if (buf_) (void)hipFree(buf_);
+    if (host_) (void)hipHostFree(host_);
 }
 
 void Selector::set(const float* w, size_t n, float threshold) {
@@ -53,6 +54,20 @@ void Selector::ensure(size_t bytes) {
     buf_size_ = bytes;
 }
 
+void Selector::ensure_host(size_t bytes) {
+    if (host_size_ >= bytes) return;
+    if (host_) (void)hipHostFree(host_);
+    host_ = nullptr;
+    host_size_ = 0;
+    void* p = nullptr;
+    hip_check(hipHostMalloc(&p, bytes, hipHostMallocDefault), "selector staging allocation");
+    host_ = static_cast<std::byte*>(p);
+    host_size_ = bytes;
+}
+
+// The problem list goes up and the results come back through pinned staging memory on
+// `st`, behind the work already queued there (which may still be writing A and B); one
+// synchronization at the end.  Both buffers are idle on entry: every call ends with it.
 void Selector::evaluate(hipStream_t st, const std::vector<SelProblem>& probs, std::vector<float>* feat,
                         std::vector<float>* prob) {
     const int n = static_cast<int>(probs.size());
@@ -61,35 +76,43 @@ void Selector::evaluate(hipStream_t st, const std::vector<SelProblem>& probs, st
     const size_t oF = oS + align_up(k::sel_scratch_bytes(n), 256);
     const size_t oO = oF + align_up(sizeof(float) * kSelFeatures * n, 256);
     const size_t total = oO + align_up(sizeof(float) * kSelOutputs * n, 256);
-    hip_check(hipStreamSynchronize(st), "stream synchronize");  // buf_ may be in use by earlier work
+    const size_t nfeat = static_cast<size_t>(n) * kSelFeatures, nout = static_cast<size_t>(n) * kSelOutputs;
+    // host staging: the problems at 0, features at hF, outputs at hO
+    const size_t hF = align_up(sizeof(SelProblem) * n, 256), hO = hF + align_up(sizeof(float) * nfeat, 256);
     ensure(total);
-    hip_check(hipMemcpy(buf_ + oP, probs.data(), sizeof(SelProblem) * n, hipMemcpyHostToDevice), "selector upload");
+    ensure_host(hO + sizeof(float) * nout);
+    std::memcpy(host_, probs.data(), sizeof(SelProblem) * n);
+    hip_check(hipMemcpyAsync(buf_ + oP, host_, sizeof(SelProblem) * n, hipMemcpyHostToDevice, st), "selector upload");
     float* F = reinterpret_cast<float*>(buf_ + oF);
     k::sel_features(st, reinterpret_cast<const SelProblem*>(buf_ + oP), n, buf_ + oS, F);
-    if (prob && loaded()) {
+    const bool run_mlp = prob && loaded();
+    if (run_mlp) {
         if (!dw_) {
             void* p = nullptr;
             hip_check(hipMalloc(&p, sizeof(float) * kSelParams), "selector weights allocation");
             dw_ = static_cast<float*>(p);
         }
         if (dirty_) {
-            hip_check(hipMemcpyAsync(dw_, w_.data(), sizeof(float) * kSelParams, hipMemcpyHostToDevice, st),
+            hip_check(hipMemcpy(dw_, w_.data(), sizeof(float) * kSelParams, hipMemcpyHostToDevice),
                       "selector weights upload");
             dirty_ = false;
         }
         k::mlp_forward(st, F, n, dw_, nullptr, nullptr, nullptr, reinterpret_cast<float*>(buf_ + oO));
     }
     hip_check(hipGetLastError(), "selector launch");
+    if (feat)
+        hip_check(hipMemcpyAsync(host_ + hF, F, sizeof(float) * nfeat, hipMemcpyDeviceToHost, st), "selector read");
+    if (run_mlp)
+        hip_check(hipMemcpyAsync(host_ + hO, buf_ + oO, sizeof(float) * nout, hipMemcpyDeviceToHost, st),
+                  "selector read");
     hip_check(hipStreamSynchronize(st), "stream synchronize");
     if (feat) {
-        feat->resize(static_cast<size_t>(n) * kSelFeatures);
-        hip_check(hipMemcpy(feat->data(), F, sizeof(float) * feat->size(), hipMemcpyDeviceToHost), "selector read");
+        const float* f = reinterpret_cast<const float*>(host_ + hF);
+        feat->assign(f, f + nfeat);
     }
     if (prob) {
-        prob->assign(static_cast<size_t>(n) * kSelOutputs, 0.f);
-        if (loaded())
-            hip_check(hipMemcpy(prob->data(), buf_ + oO, sizeof(float) * prob->size(), hipMemcpyDeviceToHost),
-                      "selector read");
+        prob->assign(nout, 0.f);
+        if (run_mlp) std::memcpy(prob->data(), host_ + hO, sizeof(float) * nout);
     }
 }
 

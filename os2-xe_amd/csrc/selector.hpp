@@ -65,12 +65,15 @@ public:
 
 private:
     void ensure(size_t bytes);
+    void ensure_host(size_t bytes);
     std::vector<float> w_;
     float tau_ = 0.9f;
     float* dw_ = nullptr;  // device copy of the weights
     bool dirty_ = true;
     std::byte* buf_ = nullptr;
     size_t buf_size_ = 0;
+    std::byte* host_ = nullptr;  // pinned staging for the problem list and the results
+    size_t host_size_ = 0;
 };
 
 }  // namespace oz2

@@ -70,26 +70,36 @@ __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __rest
     const double* X = side == 0 ? q.A : q.B;
     const int64_t ld = side == 0 ? q.lda : q.ldb;
     const bool contig = side == 0 ? q.ta != 0 : q.tb == 0;  // k runs along memory
+    // sampled position j of np: p = j k / np, which is j when np = k and j k / 2^11 when np = 2^11
+    static_assert(kSelCols == 1 << 11, "the sampled positions are a shift of j k");
+    const int64_t kk = q.k;
+    const bool every = kk <= kSelCols;
     auto at = [&](int j) -> uint64_t {
-        const int64_t pp = static_cast<int64_t>(j) * q.k / np;
+        const int64_t pp = every ? j : (static_cast<int64_t>(j) * kk) >> 11;
         const double v = contig ? X[pp + i * ld] : X[i + pp * ld];
         return static_cast<uint64_t>(__double_as_longlong(v)) & 0x7FFFFFFFFFFFFFFFull;
     };
     __shared__ unsigned int smax, sgmax;
     __shared__ unsigned long long scnt[kBins + 2];
-    __shared__ uint64_t vals[kSelCols];  // the sampled magnitudes: read from memory once
     if (threadIdx.x == 0) {
         smax = 0;
         sgmax = 0;
     }
     if (threadIdx.x < kBins + 2) scnt[threadIdx.x] = 0;
-    __syncthreads();
-    unsigned int emax = 0;
-    for (int j = threadIdx.x; j < np; j += 256) {
-        const uint64_t b = at(j);
-        vals[j] = b;
-        emax = max(emax, static_cast<unsigned int>(b >> 52));
+    // thread t holds sampled positions t, t + 256, ...: all its loads are issued before
+    // any is used (the strided side fetches one line per element), and the magnitudes
+    // stay in registers for the gap pass
+    constexpr int kPer = kSelCols / 256;
+    uint64_t v[kPer] = {};
+    if (np > 0) {
+        // positions past np read position np - 1 (in bounds) and are skipped below
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) v[u] = at(min(static_cast<int>(threadIdx.x) + 256 * u, np - 1));
     }
+    unsigned int emax = 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) emax = max(emax, static_cast<unsigned int>(v[u] >> 52));
+    __syncthreads();
     atomicMax(&smax, emax);
     __syncthreads();
     const unsigned int e0 = smax;
@@ -98,8 +108,10 @@ __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __rest
     unsigned int lc[kBins + 1];
 #pragma unroll
     for (int b = 0; b <= kBins; ++b) lc[b] = 0;
-    for (int j = threadIdx.x; j < np; j += 256) {
-        const uint64_t bits = vals[j];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        if (static_cast<int>(threadIdx.x) + 256 * u >= np) continue;
+        const uint64_t bits = v[u];
         if (bits == 0) {
             ++lc[kBins];
             continue;

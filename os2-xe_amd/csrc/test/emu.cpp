@@ -66,15 +66,21 @@ extern "C" {
 // symmetric int8 residues of RNE(v_i * 2^e_i): planes[t * n + i]; e = INT32_MIN gives 0.
 // the octets of u = 2^OFF + x: 8 bytes for tk <= 62, else 14
 void emu_split(const double* v, const int32_t* e, int64_t n, int s, int tk, int8_t* planes) {
-    if (tk > kNarrowScaleBits) split_form<math::Limbs::Octets14>(v, e, n, s, planes);
-    else split_form<math::Limbs::Octets8>(v, e, n, s, planes);
+    switch (limb_form(tk)) {
+        case 0: split_form<math::Limbs::Octets8>(v, e, n, s, planes); break;
+        case 1: split_form<math::Limbs::Octets12>(v, e, n, s, planes); break;
+        default: split_form<math::Limbs::Octets14>(v, e, n, s, planes); break;
+    }
 }
 
 // complex: groups Re, Im (negated if conj), Re+Im reduced to [-h, m-1-h]: planes[(g*s + t)*n + i]
 void emu_split_cplx(const double* re, const double* im, const int32_t* e, int64_t n, int s, int tk, int conj,
                     int8_t* planes) {
-    if (tk > kNarrowScaleBits) split_cplx_form<math::Limbs::Octets14>(re, im, e, n, s, conj, planes);
-    else split_cplx_form<math::Limbs::Octets8>(re, im, e, n, s, conj, planes);
+    switch (limb_form(tk)) {
+        case 0: split_cplx_form<math::Limbs::Octets8>(re, im, e, n, s, conj, planes); break;
+        case 1: split_cplx_form<math::Limbs::Octets12>(re, im, e, n, s, conj, planes); break;
+        default: split_cplx_form<math::Limbs::Octets14>(re, im, e, n, s, conj, planes); break;
+    }
 }
 
 // exact-quotient reduction of modulus t of s over every v < 2^23: entries that differ from v mod m

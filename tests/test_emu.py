@@ -57,7 +57,8 @@ def test_split_golden():
             assert got.tolist() == col["res"], (case["s"], case["tk"])
 
 
-@pytest.mark.parametrize("s,tk", [(2, 6), (8, 24), (13, 44), (16, 55), (18, 62), (19, 63), (24, 86), (24, 100)])
+@pytest.mark.parametrize("s,tk", [(2, 6), (8, 24), (13, 44), (16, 55), (18, 62), (19, 63), (24, 86), (24, 91),
+                                  (24, 94), (24, 95), (24, 100)])
 def test_split_random_vs_oracle(s, tk, rng):
     n = 20000
     v = rng.uniform(-1, 1, n) * np.exp2(rng.integers(-70, 1, n).astype(float))
@@ -73,7 +74,7 @@ def test_split_random_vs_oracle(s, tk, rng):
 
 
 @pytest.mark.parametrize("conj", [0, 1])
-@pytest.mark.parametrize("s,tk", [(14, 49), (16, 55), (24, 86)])
+@pytest.mark.parametrize("s,tk", [(14, 49), (16, 55), (24, 86), (24, 94), (24, 100)])
 def test_split_complex_vs_oracle(s, tk, conj, rng):
     n = 5000
     re = rng.uniform(-1, 1, n)
