@@ -52,94 +52,123 @@ struct SideAcc {
     int rowexp[kSelRows];
 };
 
-// One block per sampled row of op(A) or column of op(B).
+// Sampled row r of nr (of `rows`) is row r rows / nr; sampled position j of np = min(k, 2^11) is
+// p = j k / np, which is j when np = k and j k / 2^11 when np = 2^11.
+static_assert(kSelCols == 1 << 11, "the sampled positions are a shift of j k");
+__device__ __forceinline__ int64_t sampled_pos(int j, int64_t k) {
+    return k <= kSelCols ? j : (static_cast<int64_t>(j) * k) >> 11;
+}
+
+// One block per group of 16 consecutive sampled rows of op(A) or columns of op(B), all
+// their sampled positions.  Lanes run along memory: along k for rows contiguous in memory
+// (op(A) = T, op(B) = N; thread t: row t >> 4, positions (t & 15) + 16 i), along the rows
+// for strided ones (op(A) = N, op(B) = T; row t & 15, positions (t >> 4) + 16 i), where
+// sampled rows less than 16 apart share the lines a load fetches.  Pass 1 keeps the
+// exponent field of every sampled element (0xFFFF for a zero) in LDS, position-major,
+// and finds each row's largest; pass 2 bins the gaps to it.  The block's counts go to
+// the (problem, side) accumulator in one set of atomics (one set per row took as long as
+// all the loads).
+constexpr int kSelGroup = 16;
 __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __restrict__ probs, int nprob,
                                                         SideAcc* __restrict__ acc) {
-    const int per = 2 * kSelRows;
-    const int p = blockIdx.x / per;
-    const int rem = blockIdx.x - p * per;
-    const int side = rem / kSelRows, r = rem - side * kSelRows;
+    constexpr int groups = kSelRows / kSelGroup;
+    const int p = blockIdx.x / (2 * groups);
+    const int rem = blockIdx.x - p * 2 * groups;
+    const int side = rem / groups, g = rem - side * groups;
     if (p >= nprob) return;
     const SelProblem& q = probs[p];
     const int64_t rows = side == 0 ? q.m : q.n;
     const int nr = static_cast<int>(min<int64_t>(rows, kSelRows));
-    if (r >= nr) return;
-    const int64_t i = static_cast<int64_t>(r) * rows / nr;
     const int np = static_cast<int>(min<int64_t>(q.k, kSelCols));
+    if (g * kSelGroup >= nr || np <= 0) return;
+    const bool contig = side == 0 ? q.ta != 0 : q.tb == 0;  // k runs along memory
+    const int t = static_cast<int>(threadIdx.x);
+    const int rl = contig ? t >> 4 : t & (kSelGroup - 1), kl = contig ? t & 15 : t >> 4;
+    const int r = g * kSelGroup + rl;
+    const bool rowok = r < nr;
+    const int64_t i = rowok ? static_cast<int64_t>(r) * rows / nr : 0;  // absent rows read row 0, unused
     // element (i, p) of op(A) (side 0) or (p, i) of op(B) (side 1), column-major storage
     const double* X = side == 0 ? q.A : q.B;
     const int64_t ld = side == 0 ? q.lda : q.ldb;
-    const bool contig = side == 0 ? q.ta != 0 : q.tb == 0;  // k runs along memory
-    // sampled position j of np: p = j k / np, which is j when np = k and j k / 2^11 when np = 2^11
-    static_assert(kSelCols == 1 << 11, "the sampled positions are a shift of j k");
+    const int64_t step_k = contig ? 1 : ld, step_i = contig ? ld : 1;
+    const double* row = X + i * step_i;
     const int64_t kk = q.k;
-    const bool every = kk <= kSelCols;
-    auto at = [&](int j) -> uint64_t {
-        const int64_t pp = every ? j : (static_cast<int64_t>(j) * kk) >> 11;
-        const double v = contig ? X[pp + i * ld] : X[i + pp * ld];
-        return static_cast<uint64_t>(__double_as_longlong(v)) & 0x7FFFFFFFFFFFFFFFull;
-    };
-    __shared__ unsigned int smax, sgmax;
-    __shared__ unsigned long long scnt[kBins + 2];
-    if (threadIdx.x == 0) {
-        smax = 0;
-        sgmax = 0;
+
+    __shared__ uint16_t ex[kSelCols][kSelGroup];  // 64 KB
+    __shared__ unsigned int rmax[kSelGroup], rgmax[kSelGroup];
+    __shared__ unsigned long long bcnt[kBins + 2];
+    if (t < kSelGroup) {
+        rmax[t] = 0;
+        rgmax[t] = 0;
     }
-    if (threadIdx.x < kBins + 2) scnt[threadIdx.x] = 0;
-    // thread t holds sampled positions t, t + 256, ...: all its loads are issued before
-    // any is used (the strided side fetches one line per element), and the magnitudes
-    // stay in registers for the gap pass
-    constexpr int kPer = kSelCols / 256;
-    uint64_t v[kPer] = {};
-    if (np > 0) {
-        // positions past np read position np - 1 (in bounds) and are skipped below
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) v[u] = at(min(static_cast<int>(threadIdx.x) + 256 * u, np - 1));
-    }
+    if (t < kBins + 2) bcnt[t] = 0;
+
+    // pass 1: exponents into LDS and row maxima; 16 loads in flight per step, all at j < np
+    constexpr int U = 16;
     unsigned int emax = 0;
+    for (int j0 = kl; j0 < np; j0 += 16 * U) {
+        uint64_t v[U];
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) emax = max(emax, static_cast<unsigned int>(v[u] >> 52));
+        for (int u = 0; u < U; ++u) {
+            const int j = min(j0 + 16 * u, np - 1);
+            v[u] = static_cast<uint64_t>(__double_as_longlong(row[sampled_pos(j, kk) * step_k])) &
+                   0x7FFFFFFFFFFFFFFFull;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + 16 * u;
+            if (j < np) {
+                const unsigned int e = static_cast<unsigned int>(v[u] >> 52);
+                ex[j][rl] = v[u] == 0 ? uint16_t{0xFFFF} : static_cast<uint16_t>(e);
+                emax = max(emax, e);
+            }
+        }
+    }
     __syncthreads();
-    atomicMax(&smax, emax);
+    atomicMax(&rmax[rl], emax);
     __syncthreads();
-    const unsigned int e0 = smax;
+
+    // pass 2: gaps against the row maximum (absent rows count nothing)
+    const unsigned int e0 = rmax[rl];
     unsigned int gmax = 0;
     unsigned long long gsum = 0;
     unsigned int lc[kBins + 1];
 #pragma unroll
     for (int b = 0; b <= kBins; ++b) lc[b] = 0;
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        if (static_cast<int>(threadIdx.x) + 256 * u >= np) continue;
-        const uint64_t bits = v[u];
-        if (bits == 0) {
+    for (int j = kl; rowok && j < np; j += 16) {
+        const unsigned int e = ex[j][rl];
+        if (e == 0xFFFF) {
             ++lc[kBins];
             continue;
         }
-        const unsigned int e = static_cast<unsigned int>(bits >> 52);
-        const int g = e >= 0x7FF || e0 >= 0x7FF ? 0 : static_cast<int>(e0 - e);
-        gmax = max(gmax, static_cast<unsigned int>(g));
-        gsum += static_cast<unsigned long long>(g);
-        const int bin = gap_bin(g);
+        const int gp = e >= 0x7FF || e0 >= 0x7FF ? 0 : static_cast<int>(e0 - e);
+        gmax = max(gmax, static_cast<unsigned int>(gp));
+        gsum += static_cast<unsigned long long>(gp);
+        const int bin = gap_bin(gp);
 #pragma unroll
         for (int b = 0; b < kBins; ++b) lc[b] += b == bin ? 1u : 0u;
     }
 #pragma unroll
     for (int b = 0; b <= kBins; ++b)
-        if (lc[b]) atomicAdd(&scnt[b], static_cast<unsigned long long>(lc[b]));
-    if (gsum) atomicAdd(&scnt[kBins + 1], gsum);
-    atomicMax(&sgmax, gmax);
+        if (lc[b]) atomicAdd(&bcnt[b], static_cast<unsigned long long>(lc[b]));
+    if (gsum) atomicAdd(&bcnt[kBins + 1], gsum);
+    atomicMax(&rgmax[rl], gmax);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        SideAcc& a = acc[2 * p + side];
+    SideAcc& a = acc[2 * p + side];
+    if (t < kSelGroup && g * kSelGroup + t < nr) a.rowexp[g * kSelGroup + t] = static_cast<int>(rmax[t]);
+    if (t == 0) {
 #pragma unroll
         for (int b = 0; b < kBins; ++b)
-            if (scnt[b]) atomicAdd(&a.cnt[b], scnt[b]);
-        if (scnt[kBins]) atomicAdd(&a.zeros, scnt[kBins]);
-        if (scnt[kBins + 1]) atomicAdd(&a.gap_sum, scnt[kBins + 1]);
-        atomicMax(&a.gap_max, sgmax);
-        if (sgmax >= 24) atomicAdd(&a.wide_rows, 1u);
-        a.rowexp[r] = static_cast<int>(e0);
+            if (bcnt[b]) atomicAdd(&a.cnt[b], bcnt[b]);
+        if (bcnt[kBins]) atomicAdd(&a.zeros, bcnt[kBins]);
+        if (bcnt[kBins + 1]) atomicAdd(&a.gap_sum, bcnt[kBins + 1]);
+        unsigned int gm = 0, wide = 0;
+        for (int w = 0; w < kSelGroup && g * kSelGroup + w < nr; ++w) {
+            gm = max(gm, rgmax[w]);
+            wide += rgmax[w] >= 24 ? 1u : 0u;
+        }
+        atomicMax(&a.gap_max, gm);
+        if (wide) atomicAdd(&a.wide_rows, wide);
     }
 }
 
@@ -164,48 +193,59 @@ __device__ float hist_quantile(const unsigned long long* cnt, unsigned long long
 }
 
 // One thread per problem: 48 features.
-__global__ void sel_finalize_kernel(const SelProblem* __restrict__ probs, int nprob, const SideAcc* __restrict__ acc,
-                                    float* __restrict__ feat) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// One 128-thread block per problem: the row exponents of both sides come into LDS with one
+// load per thread, then thread `side` reduces its side in row order and writes its
+// features (thread 0 also the shape features).
+__global__ void __launch_bounds__(kSelRows) sel_finalize_kernel(const SelProblem* __restrict__ probs, int nprob,
+                                                                const SideAcc* __restrict__ acc,
+                                                                float* __restrict__ feat) {
+    const int p = blockIdx.x, t = static_cast<int>(threadIdx.x);
     if (p >= nprob) return;
     const SelProblem& q = probs[p];
+    __shared__ int rowexp[2][kSelRows];
+    rowexp[0][t] = acc[2 * p].rowexp[t];
+    rowexp[1][t] = acc[2 * p + 1].rowexp[t];
+    __syncthreads();
+    if (t >= 2) return;
     float* f = feat + static_cast<int64_t>(p) * kSelFeatures;
     const float lk = log2f(static_cast<float>(max<int64_t>(q.k, 1)));
-    f[0] = lk / 16.f;
-    f[1] = static_cast<float>(q.bits) / 64.f;
-    f[2] = log2f(static_cast<float>(max<int64_t>(q.m, 1))) / 16.f;
-    f[3] = log2f(static_cast<float>(max<int64_t>(q.n, 1))) / 16.f;
-    for (int side = 0; side < 2; ++side) {
-        const SideAcc& a = acc[2 * p + side];
-        unsigned long long nz = 0;
-        for (int b = 0; b < kBins; ++b) nz += a.cnt[b];
-        const unsigned long long all = nz + a.zeros;
-        float* h = f + 4 + 14 * side;
-        for (int b = 0; b < kBins; ++b) h[b] = nz ? static_cast<float>(static_cast<double>(a.cnt[b]) / nz) : 0.f;
-        h[kBins] = all ? static_cast<float>(static_cast<double>(a.zeros) / all) : 0.f;
-        float* g = f + 32 + 4 * side;
-        g[0] = nz ? static_cast<float>(static_cast<double>(a.gap_sum) / nz) / 32.f : 0.f;
-        g[1] = static_cast<float>(min(a.gap_max, 128u)) / 64.f;
-        g[2] = hist_quantile(a.cnt, nz, 0.90f) / 64.f;
-        g[3] = hist_quantile(a.cnt, nz, 0.99f) / 64.f;
-        const int64_t rows = side == 0 ? q.m : q.n;
-        const int nr = static_cast<int>(min<int64_t>(rows, kSelRows));
-        int lo = 1 << 30, hi = 0;
-        double mean = 0;
-        for (int r = 0; r < nr; ++r) {
-            lo = min(lo, a.rowexp[r]);
-            hi = max(hi, a.rowexp[r]);
-            mean += a.rowexp[r];
-        }
-        mean = nr ? mean / nr : 0;
-        double mad = 0;
-        for (int r = 0; r < nr; ++r) mad += fabs(a.rowexp[r] - mean);
-        f[40 + 2 * side] = nr ? static_cast<float>(hi - lo) / 64.f : 0.f;
-        f[41 + 2 * side] = nr ? static_cast<float>(mad / nr) / 16.f : 0.f;
-        f[44 + side] = nr ? static_cast<float>(a.wide_rows) / nr : 0.f;
+    if (t == 0) {
+        f[0] = lk / 16.f;
+        f[1] = static_cast<float>(q.bits) / 64.f;
+        f[2] = log2f(static_cast<float>(max<int64_t>(q.m, 1))) / 16.f;
+        f[3] = log2f(static_cast<float>(max<int64_t>(q.n, 1))) / 16.f;
+        f[46] = lk * static_cast<float>(q.bits) / 1024.f;
+        f[47] = 1.f;
     }
-    f[46] = lk * static_cast<float>(q.bits) / 1024.f;
-    f[47] = 1.f;
+    const int side = t;
+    const SideAcc& a = acc[2 * p + side];
+    unsigned long long nz = 0;
+    for (int b = 0; b < kBins; ++b) nz += a.cnt[b];
+    const unsigned long long all = nz + a.zeros;
+    float* h = f + 4 + 14 * side;
+    for (int b = 0; b < kBins; ++b) h[b] = nz ? static_cast<float>(static_cast<double>(a.cnt[b]) / nz) : 0.f;
+    h[kBins] = all ? static_cast<float>(static_cast<double>(a.zeros) / all) : 0.f;
+    float* g = f + 32 + 4 * side;
+    g[0] = nz ? static_cast<float>(static_cast<double>(a.gap_sum) / nz) / 32.f : 0.f;
+    g[1] = static_cast<float>(min(a.gap_max, 128u)) / 64.f;
+    g[2] = hist_quantile(a.cnt, nz, 0.90f) / 64.f;
+    g[3] = hist_quantile(a.cnt, nz, 0.99f) / 64.f;
+    const int64_t rows = side == 0 ? q.m : q.n;
+    const int nr = static_cast<int>(min<int64_t>(rows, kSelRows));
+    const int* re = rowexp[side];
+    int lo = 1 << 30, hi = 0;
+    double mean = 0;
+    for (int r = 0; r < nr; ++r) {
+        lo = min(lo, re[r]);
+        hi = max(hi, re[r]);
+        mean += re[r];
+    }
+    mean = nr ? mean / nr : 0;
+    double mad = 0;
+    for (int r = 0; r < nr; ++r) mad += fabs(re[r] - mean);
+    f[40 + 2 * side] = nr ? static_cast<float>(hi - lo) / 64.f : 0.f;
+    f[41 + 2 * side] = nr ? static_cast<float>(mad / nr) / 16.f : 0.f;
+    f[44 + side] = nr ? static_cast<float>(a.wide_rows) / nr : 0.f;
 }
 
 // ---- MLP ------------------------------------------------------------------------
@@ -329,9 +369,10 @@ void sel_features(hipStream_t st, const SelProblem* probs, int nprob, void* scra
     if (nprob <= 0) return;
     SideAcc* acc = static_cast<SideAcc*>(scratch);
     hipLaunchKernelGGL(sel_clear_kernel, dim3(64), dim3(256), 0, st, acc, 2 * nprob);
-    hipLaunchKernelGGL(sel_stats_kernel, dim3(static_cast<unsigned>(nprob) * 2 * kSelRows), dim3(256), 0, st, probs,
-                       nprob, acc);
-    hipLaunchKernelGGL(sel_finalize_kernel, dim3((nprob + 63) / 64), dim3(64), 0, st, probs, nprob, acc, feat);
+    hipLaunchKernelGGL(sel_stats_kernel, dim3(static_cast<unsigned>(nprob) * 2 * (kSelRows / kSelGroup)), dim3(256),
+                       0, st, probs, nprob, acc);
+    hipLaunchKernelGGL(sel_finalize_kernel, dim3(static_cast<unsigned>(nprob)), dim3(kSelRows), 0, st, probs, nprob,
+                       acc, feat);
 }
 
 void mlp_forward(hipStream_t st, const float* X, int nb, const float* W, float* H1, float* H2, float* Z, float* P) {

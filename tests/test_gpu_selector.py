@@ -29,6 +29,8 @@ FEATURE_CASES = [  # ta, tb, m, n, k, dist
     ("T", "T", 129, 70, 5000, "wide"),
     ("N", "N", 200, 257, 3000, "ints"),
     ("N", "T", 64, 64, 64, "sparse"),
+    ("N", "N", 2500, 40, 300, "wide"),    # strided sampled rows 19-20 apart: no two share a line
+    ("N", "T", 1000, 20, 2500, "phi"),    # 7-8 apart, k subsampled; op(B) strided, 20 rows: a partial group
 ]
 
 
