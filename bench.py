@@ -187,6 +187,12 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
                    persistent=tune["persistent"], largest_first=tune["largest_first"])
     t_group = timed(b2.run)
     b2.close()
+    # the chosen plan with the other GEMM schedule (persistent workgroups pulling tiles from a
+    # counter for the 128- and 64-row groups; 256-wide groups keep one workgroup per tile)
+    b3 = oz2.Batch(oz, descs, bits=bits, graph=True, select=True, tiles=tune["tiles"],
+                   persistent=not tune["persistent"], largest_first=tune["largest_first"])
+    t_other = timed(b3.run)
+    b3.close()
     oz.set_profiling(True)
     out = {
         "workload": f"{count} DGEMMs N/N, m n k log-uniform [64,4096], half U[-1,1) half (1+U)*2^e e in [-12,12], "
@@ -202,7 +208,9 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
                                   "autotuned": {"persistent": tune["persistent"], "largest_first": tune["largest_first"],
                                                 "tiles": {("256x128" if t == 256128 else str(t)): tune["tiles"].count(t)
                                                           for t in (64, 128, 256, 256128)},
-                                                "tune_ms_no_graph": round(tune["ms"], 3)}},
+                                                "tune_ms_no_graph": round(tune["ms"], 3)},
+                                  "other_schedule": {"persistent": not tune["persistent"],
+                                                     "ms": round(t_other * 1e3, 3)}},
         "speedup_vs_plain": round(t_plain / t_batch, 3),
         "selector": {"ms": round(t_sel * 1e3, 3),
                      "what": "features of all problems + MLP + choice through the Python API (oz2_selector_choose, "

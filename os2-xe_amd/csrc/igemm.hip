@@ -367,9 +367,13 @@ void prep_gemm(std::vector<GroupedLaunch>& out, const std::vector<GemmJob>& jobs
         if (g.n) out.push_back(std::move(g));
 }
 
-// the persistent loop around a 256-wide tile spills registers (its loop state on top
-// of 256 accumulators per lane), so those groups always take one workgroup per tile
-static bool persistent_form(Tile t, bool persistent) { return persistent && (t == Tile::T128 || t == Tile::T64); }
+// Persistent forms: 128 x 128 and 64 x 64 keep their workgroups per CU; 256 x 128 runs one
+// workgroup per CU there (with two, its loop state on top of the 252 registers the tile
+// needs spills 23 dwords; with one it has the accumulation registers to itself: 194 + 128,
+// no spill).  The loop around a 256 x 256 tile spills with one (256 accumulators per
+// lane), so those groups always take one workgroup per tile.
+static bool persistent_form(Tile t, bool persistent) { return persistent && t != Tile::T256; }
+static int persistent_blocks_per_cu(Tile t) { return t == Tile::T256x128 ? 1 : tile_blocks_per_cu(t); }
 
 int gemm_grouped_launches(const GroupedLaunch& g, bool persistent) {
     if (g.n == 0) return 0;
@@ -382,11 +386,12 @@ void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* count
     const GemmArgs* items = static_cast<const GemmArgs*>(g.d_items);
     if (persistent_form(t, persistent)) {
         hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, counter, 1);
-        const uint32_t grid = std::min<uint32_t>(g.blocks, static_cast<uint32_t>(cus * tile_blocks_per_cu(t)));
+        const uint32_t grid = std::min<uint32_t>(g.blocks, static_cast<uint32_t>(cus * persistent_blocks_per_cu(t)));
 #define OZ2_LAUNCH(BM, BN, MINB, NS)                                                                             \
     hipLaunchKernelGGL((igemm_grouped_kernel<BM, BN, MINB, NS>), dim3(grid), dim3(256), 0, st, items, g.d_start, \
                        g.d_owner, g.blocks, counter)
         if (t == Tile::T128) OZ2_LAUNCH(128, 128, 2, 4);
+        else if (t == Tile::T256x128) OZ2_LAUNCH(256, 128, 1, 4);
         else OZ2_LAUNCH(64, 64, 4, 4);
 #undef OZ2_LAUNCH
         return;

@@ -67,7 +67,9 @@ void Selector::ensure_host(size_t bytes) {
 
 // The problem list goes up and the results come back through pinned staging memory on
 // `st`, behind the work already queued there (which may still be writing A and B); one
-// synchronization at the end.  Both buffers are idle on entry: every call ends with it.
+// synchronization at the end.  Both buffers are idle on entry, since a call that returns
+// normally ends with that synchronization; one that threw after queueing work is waited
+// for first (pending_).
 void Selector::evaluate(hipStream_t st, const std::vector<SelProblem>& probs, std::vector<float>* feat,
                         std::vector<float>* prob) {
     const int n = static_cast<int>(probs.size());
@@ -79,9 +81,14 @@ void Selector::evaluate(hipStream_t st, const std::vector<SelProblem>& probs, st
     const size_t nfeat = static_cast<size_t>(n) * kSelFeatures, nout = static_cast<size_t>(n) * kSelOutputs;
     // host staging: the problems at 0, features at hF, outputs at hO
     const size_t hF = align_up(sizeof(SelProblem) * n, 256), hO = hF + align_up(sizeof(float) * nfeat, 256);
+    if (pending_) {  // the stream of that call may be gone: wait for the device
+        hip_check(hipDeviceSynchronize(), "device synchronize");
+        pending_ = false;
+    }
     ensure(total);
     ensure_host(hO + sizeof(float) * nout);
     std::memcpy(host_, probs.data(), sizeof(SelProblem) * n);
+    pending_ = true;
     hip_check(hipMemcpyAsync(buf_ + oP, host_, sizeof(SelProblem) * n, hipMemcpyHostToDevice, st), "selector upload");
     float* F = reinterpret_cast<float*>(buf_ + oF);
     k::sel_features(st, reinterpret_cast<const SelProblem*>(buf_ + oP), n, buf_ + oS, F);
@@ -106,6 +113,7 @@ void Selector::evaluate(hipStream_t st, const std::vector<SelProblem>& probs, st
         hip_check(hipMemcpyAsync(host_ + hO, buf_ + oO, sizeof(float) * nout, hipMemcpyDeviceToHost, st),
                   "selector read");
     hip_check(hipStreamSynchronize(st), "stream synchronize");
+    pending_ = false;
     if (feat) {
         const float* f = reinterpret_cast<const float*>(host_ + hF);
         feat->assign(f, f + nfeat);

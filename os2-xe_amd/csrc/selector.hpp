@@ -74,6 +74,7 @@ private:
     size_t buf_size_ = 0;
     std::byte* host_ = nullptr;  // pinned staging for the problem list and the results
     size_t host_size_ = 0;
+    bool pending_ = false;       // a call queued work and did not wait for it (it threw)
 };
 
 }  // namespace oz2

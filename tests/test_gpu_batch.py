@@ -181,17 +181,19 @@ def test_batch_argument_errors(oz):
         oz2.Batch(oz, d, tiles=[256, 96])
 
 
-@pytest.mark.parametrize("persistent,largest_first", [(True, True), (False, False), (True, False)])
-def test_schedules_identical(oz, persistent, largest_first):
-    log(f"batch schedule persistent={persistent} largest_first={largest_first}")
+@pytest.mark.parametrize("persistent,largest_first,tile", [(True, True, None), (False, False, None),
+                                                           (True, False, None), (True, True, 256128),
+                                                           (True, False, 128), (True, True, 64)])
+def test_schedules_identical(oz, persistent, largest_first, tile):
+    log(f"batch schedule persistent={persistent} largest_first={largest_first} tile={tile}")
     ps = build(PROBLEMS, seed=13)
     ref = [reference(oz, p) for p in ps]
     Cs = [to_dev(p["C0"].copy()) for p in ps]
     b = oz2.Batch(oz, [desc(p, C) for p, C in zip(ps, Cs)], graph=True, persistent=persistent,
-                  largest_first=largest_first)
+                  largest_first=largest_first, tiles=None if tile is None else [tile] * len(ps))
     b.run()
     torch.cuda.synchronize()
-    check_same(ref, Cs, f"persistent={persistent} largest_first={largest_first}", ps)
+    check_same(ref, Cs, f"persistent={persistent} largest_first={largest_first} tile={tile}", ps)
     b.close()
 
 
