@@ -194,7 +194,7 @@ BatchPlan::BatchPlan(Engine& eng, std::vector<GemmDesc> descs, const std::vector
         for (const k::GroupedLaunch& g : *v)
             tables += align_up(g.items.size(), 256) + align_up(g.start.size() * 4, 256) +
                       align_up(g.owner.size() * 4, 256);
-    const size_t ncounters = gemm_.size();
+    const size_t ncounters = gemm_.size() * k::kGemmCounters;
     ws_size_ = data_bytes + tables + align_up(std::max<size_t>(1, ncounters) * 4, 256) + 4096;
     if (ws_size_ > eng_.budget())
         throw std::runtime_error("oz2: not enough device memory for the batch workspace (" +
@@ -271,7 +271,8 @@ void BatchPlan::enqueue(hipStream_t st) {
     for (const k::GroupedLaunch& g : scale_) k::launch_scale_split_grouped(st, g);
     for (const k::GroupedLaunch& g : split_) k::launch_scale_split_grouped(st, g);
     for (size_t i = 0; i < gemm_.size(); ++i)
-        k::launch_gemm_grouped(st, gemm_[i], counters_ + i, eng_.compute_units(), opt_.persistent);
+        k::launch_gemm_grouped(st, gemm_[i], counters_ + k::kGemmCounters * i, eng_.compute_units(),
+                               opt_.persistent);
     for (const k::GroupedLaunch& g : crt_) k::launch_crt_grouped(st, g);
 }
 
@@ -283,7 +284,8 @@ void BatchPlan::run(hipStream_t st) {
 
 void BatchPlan::run_gemm_only(hipStream_t st) {
     for (size_t i = 0; i < gemm_.size(); ++i)
-        k::launch_gemm_grouped(st, gemm_[i], counters_ + i, eng_.compute_units(), opt_.persistent);
+        k::launch_gemm_grouped(st, gemm_[i], counters_ + k::kGemmCounters * i, eng_.compute_units(),
+                               opt_.persistent);
     hip_check(hipGetLastError(), "batch gemm launch");
 }
 

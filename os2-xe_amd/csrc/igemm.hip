@@ -311,20 +311,36 @@ __global__ void __launch_bounds__(256, MINB) igemm_int32_kernel(GemmArgs p) {
 
 // Grouped forms over many problems; owner[w] is the item (problem, k-chunk) of
 // work index w and start[] its first index, items listed largest first.
-// Persistent: a fixed grid takes work indices from a counter until all are done.
+// Persistent: a fixed grid takes work indices from counters until all are done.  The
+// indices are split over the 8 XCDs in contiguous ranges, as in the one-tile-per-block
+// form (workgroup b runs on XCD b % 8), so the tiles that share A and B bands meet in
+// one L2; each range has a counter of its own (counter[0..7]), and a workgroup whose
+// range is done takes work from the next ranges in turn.  Every counter only grows, so
+// each workgroup leaves after at most 8 empty draws.
 template <int BM, int BN, int MINB, int NS>
 __global__ void __launch_bounds__(256, MINB) igemm_grouped_kernel(const GemmArgs* __restrict__ items,
                                                                   const uint32_t* __restrict__ start,
                                                                   const uint32_t* __restrict__ owner,
                                                                   uint32_t total, uint32_t* __restrict__ counter) {
-    __shared__ v4i smem[TileShape<BM, BN, NS>::SMEM / 16];
-    __shared__ uint32_t next;
+    // the work index is broadcast through one more 16-byte slot of the tile buffer: with a
+    // second __shared__ object the compiler cannot tell which LDS the tile's DMA writes and
+    // waits for all of it (vmcnt(0)) before every fragment read of the k-loop
+    __shared__ v4i smem[TileShape<BM, BN, NS>::SMEM / 16 + 1];
+    uint32_t& next = *reinterpret_cast<uint32_t*>(smem + TileShape<BM, BN, NS>::SMEM / 16);
+    const uint32_t x0 = blockIdx.x & 7;
+    uint32_t tried = 0;  // ranges found empty, from x0 on
     for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+        const uint32_t x = (x0 + tried) & 7;
+        const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(total) * x / 8);
+        const uint32_t hi = static_cast<uint32_t>(static_cast<uint64_t>(total) * (x + 1) / 8);
+        if (threadIdx.x == 0) next = lo + atomicAdd(&counter[x], 1u);
         __syncthreads();
         const uint32_t w = __builtin_amdgcn_readfirstlane(next);  // uniform: item fields stay in SGPRs
         __syncthreads();  // also: the previous tile's epilogue is done with LDS
-        if (w >= total) break;
+        if (w >= hi) {
+            if (++tried == 8) break;
+            continue;
+        }
         const uint32_t it = owner[w];
         const GemmArgs& p = items[it];
         uint32_t plane, tm, tn;
@@ -385,7 +401,7 @@ void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* count
     const Tile t = static_cast<Tile>(g.kind);
     const GemmArgs* items = static_cast<const GemmArgs*>(g.d_items);
     if (persistent_form(t, persistent)) {
-        hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, counter, 1);
+        hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, counter, kGemmCounters);
         const uint32_t grid = std::min<uint32_t>(g.blocks, static_cast<uint32_t>(cus * persistent_blocks_per_cu(t)));
 #define OZ2_LAUNCH(BM, BN, MINB, NS)                                                                             \
     hipLaunchKernelGGL((igemm_grouped_kernel<BM, BN, MINB, NS>), dim3(grid), dim3(256), 0, st, items, g.d_start, \

@@ -156,8 +156,10 @@ void prep_gemm(std::vector<GroupedLaunch>& out, const std::vector<GemmJob>& jobs
 void prep_crt(std::vector<GroupedLaunch>& out, const std::vector<CrtArgs>& jobs);
 bool launch_scale_split_grouped(hipStream_t st, const GroupedLaunch& g);
 // persistent (128 x 128 and 64 x 64 tiles at their blocks per CU, 256 x 128 at one per CU):
-// min(blocks, cus x that) workgroups pull tiles from *counter (reset first); else, and always
-// for 256 x 256 tiles, one workgroup per tile (counter unused)
+// min(blocks, cus x that) workgroups pull tiles through counter[0 .. kGemmCounters) (reset
+// first; one per XCD range of the work); else, and always for 256 x 256 tiles, one workgroup
+// per tile (counters unused)
+inline constexpr int kGemmCounters = 8;
 void launch_gemm_grouped(hipStream_t st, const GroupedLaunch& g, uint32_t* counter, int cus, bool persistent);
 int gemm_grouped_launches(const GroupedLaunch& g, bool persistent);  // kernels launch_gemm_grouped enqueues
 void launch_crt_grouped(hipStream_t st, const GroupedLaunch& g);
