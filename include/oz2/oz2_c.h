@@ -103,8 +103,8 @@ int oz2_moduli(int num_moduli, uint16_t* out);
  * dimensions, num_moduli).  num_moduli > 0 fixes a problem's moduli; num_moduli <= 0 lets the plan
  * choose: the learned selector when OZ2_BATCH_SELECT is set (oz2_selector_choose), else
  * moduli_for_bits(k, bits).  The plan owns one workspace for all problems and
- * a grouped launch sequence (scale, split, one persistent INT8 GEMM per macro-tile shape, CRT: about
- * ten launches for any count); with OZ2_BATCH_GRAPH that sequence is captured in a hipGraph when the
+ * a grouped launch sequence (scale, split, one INT8 GEMM per macro-tile shape, CRT: about ten
+ * launches for any count); with OZ2_BATCH_GRAPH that sequence is captured in a hipGraph when the
  * plan is created and oz2_batch_run replays it.  Buffers are bound at creation: running the plan
  * again recomputes from their current contents.  The moduli chosen at creation (by the selector or
  * moduli_for_bits) stay fixed: a run over new contents keeps them, even where the new data would need
@@ -128,8 +128,8 @@ typedef struct {
 } oz2_gemm_desc_t;
 typedef struct oz2_batch_s* oz2_batch;
 /* GRAPH: capture the launch sequence in a hipGraph; SELECT: learned num_moduli for entries with 0;
- * PERSISTENT: persistent GEMM schedule (a fixed grid pulling tiles) for 128 and 64 tiles, else one
- *   workgroup per tile (256-wide tiles always take one workgroup per tile);
+ * PERSISTENT: persistent GEMM schedule (a fixed grid drawing tiles from per-XCD work ranges) for
+ *   128, 64 and 256128 tiles, else one workgroup per tile (256 tiles always take one per tile);
  * PROBLEM_ORDER: GEMM work in problem order, else largest first */
 enum { OZ2_BATCH_GRAPH = 1, OZ2_BATCH_SELECT = 2, OZ2_BATCH_PERSISTENT = 4, OZ2_BATCH_PROBLEM_ORDER = 8 };
 int oz2_batch_create(oz2_handle h, int count, const oz2_gemm_desc_t* descs, int bits, const int* tiles, int flags,

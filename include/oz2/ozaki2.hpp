@@ -181,17 +181,36 @@ private:
     TileInfo tiling_{};
 };
 
+// The fastest plan oz2_batch_autotune measured for a list of problems: a macro tile per problem
+// and the schedule bits (OZ2_BATCH_PERSISTENT, OZ2_BATCH_PROBLEM_ORDER) to build it with.
+struct BatchTuning {
+    std::vector<int> tiles;
+    int schedule = 0;
+    double ms = 0;
+};
+
 // A plan for many DGEMMs (oz2_batch_*): one grouped launch sequence, optionally replayed from a
 // hipGraph; every problem's C equals Ozaki2::gemm with the same num_moduli.  Descriptors with
 // num_moduli <= 0 get the selector's choice (select = true) or moduli_for_bits(k, bits).
+// schedule: OZ2_BATCH_PERSISTENT and/or OZ2_BATCH_PROBLEM_ORDER (as autotune() returns them).
 class Batch {
 public:
     Batch(Ozaki2& oz, const std::vector<oz2_gemm_desc_t>& descs, int bits = 53, bool graph = true,
-          bool select = false, const std::vector<int>& tiles = {}) {
-        const int flags = (graph ? OZ2_BATCH_GRAPH : 0) | (select ? OZ2_BATCH_SELECT : 0);
+          bool select = false, const std::vector<int>& tiles = {}, int schedule = 0) {
+        const int flags = (graph ? OZ2_BATCH_GRAPH : 0) | (select ? OZ2_BATCH_SELECT : 0) |
+                          (schedule & (OZ2_BATCH_PERSISTENT | OZ2_BATCH_PROBLEM_ORDER));
         detail::check(oz2_batch_create(oz.native_handle(), static_cast<int>(descs.size()), descs.data(), bits,
                                        tiles.empty() ? nullptr : tiles.data(), flags, &b_));
         n_ = static_cast<int>(descs.size());
+    }
+    // measures candidate plans for these problems (runs them, overwriting C) and returns the fastest
+    static BatchTuning autotune(Ozaki2& oz, const std::vector<oz2_gemm_desc_t>& descs, int bits = 53,
+                                bool select = false) {
+        BatchTuning t;
+        t.tiles.assign(descs.size(), 0);
+        detail::check(oz2_batch_autotune(oz.native_handle(), static_cast<int>(descs.size()), descs.data(), bits,
+                                         select ? OZ2_BATCH_SELECT : 0, t.tiles.data(), &t.schedule, &t.ms));
+        return t;
     }
     ~Batch() {
         if (b_) (void)oz2_batch_destroy(b_);

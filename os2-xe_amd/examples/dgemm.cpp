@@ -46,13 +46,22 @@ int main(int argc, char** argv) {
     if (hipMemcpy(hC.data(), dC, hC.size() * 8, hipMemcpyDeviceToHost)) return 1;
     std::printf("C[0] = %.17g\n", hC[0]);
 
-    // the same product twice more as a batch (two problems sharing A and B): num_moduli = 0
-    // lets the learned selector choose from the data; the plan is replayed from a hipGraph
-    const oz2_gemm_desc_t d{OZ2_TRANS_N, OZ2_TRANS_N, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, 0};
-    oz2::Batch batch(oz, {d, d}, 53, /*graph=*/true, /*select=*/true);
+    // the same product and one over the first half of k as a batch: num_moduli = 0 lets the
+    // learned selector choose from the data, the autotuner picks the tiles and the GEMM
+    // schedule, and the plan is replayed from a hipGraph
+    double* dC2 = nullptr;
+    if (hipMalloc(&dC2, hC.size() * 8)) return 1;
+    const std::vector<oz2_gemm_desc_t> d = {
+        {OZ2_TRANS_N, OZ2_TRANS_N, m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m, 0},
+        {OZ2_TRANS_N, OZ2_TRANS_N, m, n, k / 2, 1.0, dA, m, dB, k, 0.0, dC2, m, 0}};
+    const oz2::BatchTuning tune = oz2::Batch::autotune(oz, d, 53, /*select=*/true);
+    oz2::Batch batch(oz, d, 53, /*graph=*/true, /*select=*/true, tune.tiles, tune.schedule);
     batch.run();
     if (hipStreamSynchronize(stream)) return 1;
-    std::printf("batch: learned s = %d (moduli_for_bits: %d)\n", batch.moduli()[0], s);
+    std::printf("batch: learned s = %d, %d (moduli_for_bits: %d); tiles %d, %d, %s schedule, %.3f ms\n",
+                batch.moduli()[0], batch.moduli()[1], s, tune.tiles[0], tune.tiles[1],
+                (tune.schedule & OZ2_BATCH_PERSISTENT) ? "persistent" : "one-tile-per-workgroup", tune.ms);
+    (void)hipFree(dC2);
 
     // the reference's comparison baseline, Ozaki Scheme I with 9 slices (45 INT8 GEMMs, 61 bits)
     oz2::Ozaki1 oz1(oz);

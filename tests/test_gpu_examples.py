@@ -67,4 +67,5 @@ def test_profiling_example(tmp_path):
 def test_dgemm_example():
     out = run("dgemm", "300", "200", "100")
     assert f"moduli s={O.moduli_for_bits(100, 53)}" in out and "batch: learned s =" in out
+    assert " schedule, " in out  # the autotuned plan ran through the C++ wrapper
     assert "ozaki1: L=9, 45 INT8 GEMMs, 61 bits per element" in out
