@@ -105,8 +105,18 @@ def accuracy(oz, torch, dA, dB, m, n, k, s):
     aH = np.abs(H)
     ulp = np.where(H == 0, np.float64(5e-324), np.nextafter(aH, np.inf) - aH)
     eu = err / ulp
+    from oz2 import check
+    tk = oz2_precision_bits(k, s)
     return {"ulp_max": float(eu.max()), "ulp_mean": float(eu.mean()), "cr_frac": float(np.mean(C == H + L)),
+            "bound_ratio": check.bound_ratio(m, n, k, dA, m, dB, k, dC, m, hi, lo, tk),
+            "bound": f"largest error / the reference's rigorous element-wise bound at s={s} (tk={tk}); <= 1 "
+                     "means every entry is within it",
             "reference": "double-double GEMM on the GPU, same inputs, alpha=1 beta=0"}
+
+
+def oz2_precision_bits(k, s):
+    import oz2
+    return oz2.Ozaki2.precision_bits(k, s)
 
 
 def make_batch(torch, count, seed):
@@ -122,6 +132,22 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
     flops = sum(2.0 * p["m"] * p["n"] * p["k"] for p in probs)
     refs = [check.dd_gemm("N", "N", p["m"], p["n"], p["k"], p["A"], p["lda"], p["B"], p["ldb"]) for p in probs]
     torch.cuda.synchronize()
+
+    def accuracy(svals):
+        """ulp_max, ulp_mean and the correctly rounded fraction over every entry of every problem, and
+        the largest ratio of an error to the reference's rigorous bound at that problem's s"""
+        emax, esum, ncr, cnt, br = 0.0, 0.0, 0.0, 0, 0.0
+        for p, (hi, lo), sv in zip(probs, refs, svals):
+            mt = check.metrics(p["C"], hi, lo, p["m"], p["ldc"])
+            ne = hi.numel()
+            emax = max(emax, mt["ulp_max"])
+            esum += mt["ulp_mean"] * ne
+            ncr += mt["cr_frac"] * ne
+            cnt += ne
+            tk = oz2.Ozaki2.precision_bits(p["k"], sv)
+            br = max(br, check.bound_ratio(p["m"], p["n"], p["k"], p["A"], p["lda"], p["B"], p["ldb"], p["C"],
+                                           p["ldc"], hi, lo, tk))
+        return {"ulp_max": emax, "ulp_mean": esum / max(cnt, 1), "cr_frac": ncr / max(cnt, 1), "bound_ratio": br}
 
     def max_ulp():
         return max(float(check.ulp_errors(p["C"], hi, lo, p["m"], p["ldc"]).max()) for p, (hi, lo) in zip(probs, refs))
@@ -144,7 +170,8 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
     oz.set_profiling(False)
     s_mfb = [oz2.Ozaki2.moduli_for_bits(p["k"], bits) for p in probs]
     t_plain = timed(lambda: plain(s_mfb))
-    ulp_plain = max_ulp()
+    acc_plain = accuracy(s_mfb)
+    ulp_plain = acc_plain["ulp_max"]
     descs = [dict(p, num_moduli=0) for p in probs]
     # the selector's own cost (exponent features of every problem + the network, two stream
     # synchronizations): paid when a plan is made or re-selected, not by its replays
@@ -159,7 +186,8 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
     b = oz2.Batch(oz, descs, bits=bits, graph=True, select=True, tiles=tune["tiles"],
                   persistent=tune["persistent"], largest_first=tune["largest_first"])
     t_batch = timed(b.run)
-    ulp_batch = max_ulp()
+    acc_batch = accuracy(b.moduli())
+    ulp_batch = acc_batch["ulp_max"]
     # re-selection of the plan's moduli from the buffers' current contents (oz2_batch_reselect:
     # features + network + choice; the contents are unchanged, so no plan is rebuilt)
     b.reselect()
@@ -179,6 +207,7 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
         u = max_ulp()
         if u <= ulp_batch:
             s_eq, ulp_eq = sv, u
+            acc_eq = accuracy([sv] * len(probs))
             t_eq = timed(lambda: plain([sv] * len(probs)))
             break
     b.close()
@@ -199,10 +228,14 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
                     f"alpha=1 beta=0, {bits}-bit target",
         "fp64_flops": flops,
         "plain_fixed_s": {"tflops": round(flops / t_plain / 1e12, 3), "ms": round(t_plain * 1e3, 3),
-                          "s": "moduli_for_bits(k, 53) per problem", "max_ulp": ulp_plain},
+                          "s": "moduli_for_bits(k, 53) per problem", "max_ulp": ulp_plain,
+                          "ulp_mean": acc_plain["ulp_mean"], "cr_frac": acc_plain["cr_frac"],
+                          "bound_ratio": acc_plain["bound_ratio"]},
         "grouped_graph_same_s": {"tflops": round(flops / t_group / 1e12, 3), "ms": round(t_group * 1e3, 3)},
         "learned_grouped_graph": {"tflops": round(flops / t_batch / 1e12, 3), "ms": round(t_batch * 1e3, 3),
-                                  "max_ulp": ulp_batch, "mean_s": sum(s_sel) / len(s_sel),
+                                  "max_ulp": ulp_batch, "ulp_mean": acc_batch["ulp_mean"],
+                                  "cr_frac": acc_batch["cr_frac"], "bound_ratio": acc_batch["bound_ratio"],
+                                  "mean_s": sum(s_sel) / len(s_sel),
                                   "s_hist": {str(v): s_sel.count(v) for v in sorted(set(s_sel))},
                                   "launches_per_run": info["launches"], "workspace_gib": round(info["workspace_bytes"] / 2 ** 30, 2),
                                   "autotuned": {"persistent": tune["persistent"], "largest_first": tune["largest_first"],
@@ -222,7 +255,9 @@ def batch_bench(oz, torch, count, reps=5, bits=53):
                      "learned_plus_reselect_ms": round((t_batch + t_resel) * 1e3, 3)},
     }
     if s_eq is not None:
-        out["plain_equal_max_ulp"] = {"s": s_eq, "max_ulp": ulp_eq, "tflops": round(flops / t_eq / 1e12, 3),
+        out["plain_equal_max_ulp"] = {"s": s_eq, "max_ulp": ulp_eq, "ulp_mean": acc_eq["ulp_mean"],
+                                      "cr_frac": acc_eq["cr_frac"], "bound_ratio": acc_eq["bound_ratio"],
+                                      "tflops": round(flops / t_eq / 1e12, 3),
                                       "ms": round(t_eq * 1e3, 3)}
         out["speedup_at_equal_max_ulp"] = round(t_eq / t_batch, 3)
         out["speedup_at_equal_max_ulp_selection_every_run"] = round(t_eq / (t_batch + t_resel), 3)
@@ -309,8 +344,12 @@ def zgemm_line(args, torch):
     torch.cuda.synchronize()
     Cv = dC1.view(n, ldc, 2)[:, :m, :].reshape(-1)
     e = check.ulp_errors(Cv, hi, lo, 2 * m * n, 2 * m * n)
+    tk = oz2.Ozaki2.precision_bits(2 * k, s)
     out["accuracy"] = {"ulp_max": float(e.max()), "ulp_mean": float(e.mean()),
                        "cr_frac": float((Cv == hi + lo).double().mean()),
+                       "bound_ratio": check.zbound_ratio("C", "N", m, n, k, dA, lda, dB, ldb, dC1, ldc, hi, lo, tk),
+                       "bound": f"largest error / the reference's rigorous element-wise bound at s={s} (tk={tk}); "
+                                "<= 1 means every entry is within it",
                        "reference": "double-double ZGEMM on the GPU, per real component, alpha=1 beta=0"}
     if not args.no_native:
         out["native_fp64"] = native_fp64(torch, m, n, k, dA, dB, cplx=True, op_a="C", lda=lda)
@@ -321,7 +360,8 @@ def zgemm_line(args, torch):
 
 def selector_line(args, torch, which):
     """wide-exponent DGEMM (4000x8000x4000) or k-split 2048x2048x200000, entries (1+U)*(+-1)*2^e, e in [-20,20]:
-    learned s vs moduli_for_bits(k, 53), each with its max-ulp against double-double"""
+    learned s vs moduli_for_bits(k, 53), each with its max-ulp against double-double and its largest ratio to
+    the reference's rigorous element-wise bound at its s"""
     import oz2
     from oz2 import check
     m, k, n = (4000, 8000, 4000) if which == "wide4000" else (2048, 200000, 2048)
@@ -349,6 +389,8 @@ def selector_line(args, torch, which):
             torch.cuda.synchronize()
             ts.append(time.perf_counter() - t0)
         met = check.metrics(dC, hi, lo, m, m)
+        met["bound_ratio"] = check.bound_ratio(m, n, k, dA, m, dB, k, dC, m, hi, lo,
+                                               oz2.Ozaki2.precision_bits(k, sv))
         res[name] = {"s": sv, "ms": round(min(ts) * 1e3, 3), "tflops": round(2.0 * m * n * k / min(ts) / 1e12, 3),
                      **met, "tiling": oz.last_tiling().as_dict()}
     res["learned"]["selector_confident"] = bool(learned[0])

@@ -79,3 +79,70 @@ def test_dd_gemm_large_sampled():
     whi, wlo = O.dd_entries("N", "N", k, A, m, B, k, rows, cols)
     at = rows + cols * m
     assert np.array_equal(bits(hi[at]), bits(whi)) and np.array_equal(bits(lo[at]), bits(wlo))
+
+
+@pytest.fixture(scope="module")
+def oz_handle():
+    import oz2
+    h = oz2.Ozaki2()
+    yield h
+    h.close()
+
+
+@pytest.mark.parametrize("dist", ["u11", "phi", "wide"])
+def test_bound_ratio_matches_oracle(oz_handle, dist):
+    """check.bound_ratio (the bench's form of the reference's element-wise bound, on the GPU)
+    against the oracle's oz2o_bound_ratio on the same product, including a row and a column of
+    zeros (exponent 0) and a result with a planted error past the bound"""
+    rng = np.random.default_rng([7, len(dist)])
+    for (m, n, k, s) in ((37, 29, 300, 12), (130, 65, 1000, 16), (64, 40, 70, 20)):
+        log(f"bound ratio {dist} m={m} n={n} k={k} s={s}")
+        A = make_mat(m, k, m + 3, dist, rng)
+        B = make_mat(k, n, k, dist, rng)
+        A[5, :] = 0.0
+        B[:, 3] = 0.0
+        A, B = flat(A), flat(B)
+        lda, ldb, ldc = m + 3, k, m + 1
+        dA, dB = to_dev(A), to_dev(B)
+        dC = torch.zeros(ldc * n, dtype=torch.float64, device="cuda")
+        oz_handle.gemm("N", "N", m, n, k, 1.0, dA, lda, dB, ldb, 0.0, dC, ldc, s)
+        torch.cuda.synchronize()
+        hi, lo = check.dd_gemm("N", "N", m, n, k, dA, lda, dB, ldb)
+        tk = O.precision_bits(k, s)
+        for planted in (False, True):
+            if planted:  # one entry moved far past its bound
+                dC[ldc + 2] += 1e-3 * max(1.0, abs(float(dC[ldc + 2])))
+            got = check.bound_ratio(m, n, k, dA, lda, dB, ldb, dC, ldc, hi, lo, tk)
+            want = O.bound_ratio("N", "N", m, n, k, A, lda, B, ldb, 1.0, 0.0, np.zeros(ldc * n), dC.cpu().numpy(),
+                                 ldc, hi.cpu().numpy(), lo.cpu().numpy(), s)
+            assert got == pytest.approx(want, rel=1e-12), (planted, got, want)
+            assert (got > 1.0) == planted
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("C", "N"), ("T", "C")])
+def test_zbound_ratio_matches_oracle(oz_handle, ta, tb):
+    """check.zbound_ratio against the oracle's oz2o_zbound_ratio (alpha = 1, beta = 0), with a
+    planted error past the bound"""
+    rng = np.random.default_rng([11, CODE[ta], CODE[tb]])
+    for (m, n, k, s) in ((23, 31, 200, 14), (70, 40, 513, 18)):
+        log(f"zbound ratio {ta}{tb} m={m} n={n} k={k} s={s}")
+        ar, ac = (m, k) if ta == "N" else (k, m)
+        br, bc = (k, n) if tb == "N" else (n, k)
+        lda, ldb, ldc = ar + 2, br + 1, m + 3
+        A = np.ascontiguousarray(flat(make_cmat(ar, ac, lda, "phi", rng))).view(np.float64)
+        B = np.ascontiguousarray(flat(make_cmat(br, bc, ldb, "phi", rng))).view(np.float64)
+        dA, dB = to_dev(A), to_dev(B)
+        dC = torch.zeros(2 * ldc * n, dtype=torch.float64, device="cuda")
+        oz_handle.zgemm(ta, tb, m, n, k, 1.0, dA, lda, dB, ldb, 0.0, dC, ldc, s)
+        torch.cuda.synchronize()
+        hi_h, lo_h = run_dd(ta, tb, m, n, k, A, lda, B, ldb, True)
+        hi, lo = to_dev(hi_h), to_dev(lo_h)
+        tk = O.precision_bits(2 * k, s)
+        for planted in (False, True):
+            if planted:
+                dC[2 * (ldc + 1) + 1] += 1e-3 * max(1.0, abs(float(dC[2 * (ldc + 1) + 1])))
+            got = check.zbound_ratio(ta, tb, m, n, k, dA, lda, dB, ldb, dC, ldc, hi, lo, tk)
+            want = O.zbound_ratio(ta, tb, m, n, k, A, lda, B, ldb, 1 + 0j, 0j, np.zeros(2 * ldc * n),
+                                  dC.cpu().numpy(), ldc, hi_h, lo_h, s)
+            assert got == pytest.approx(want, rel=1e-12), (planted, got, want)
+            assert (got > 1.0) == planted
