@@ -3,7 +3,7 @@
 // Reference behaviour: src/ozaki2_gpu.hpp:526-664.  For each entry the s
 // residues r_t (bytes from the GEMM epilogue; k-chunks are summed mod m_t)
 // determine the exact integer C' with |C'| <= M/4:
-//   q  = rint(sum_t r_t * fp32(y_t / m_t))      (error << 1/4, so q is exact)
+//   q  = round(sum_t r_t F_t / 2^16)            (F_t = round(2^16 y_t / m_t); error < 0.05, so q is exact)
 //   C' = sum_t r_t W_t - q M                     (16-bit limbs, 32-bit sums)
 // C' is rounded once to 53 bits (sticky bit) and scaled by 2^-(D_i + E_j),
 // which rounds again only for subnormal results, as in the reference.

@@ -258,33 +258,37 @@ OZ2_HD double ldexp_(double v, int e) {
 }
 
 // Exact reconstruction of one entry from residues r_t in [0, m_t); NL = 32-bit words of M.
+// The quotient q = round(sum_t r_t y_t / m_t) is taken in 16.16 fixed point: with
+// F_t = round(2^16 y_t / m_t) the sum sum_t r_t F_t / 2^16 is within s * 255 * 2^-17 < 0.05 of
+// the exact one, which lies within 1/4 of q whenever |C'| <= M/4 (the scaling guarantees it),
+// so rounding it gives q (the reference's fp32 estimate gives the same q for the same reason).
+// The fixed-point sum is one more dot2 per modulus pair in the limb sums.
 template <int NL>
 struct CrtAcc {
     static constexpr int L = 2 * NL;
-    float u;
+    uint32_t uq;  // sum r_t F_t < 24 * 255 * 2^16 < 2^29
     uint32_t acc[L];
     OZ2_HD void init() {
-        u = 0.0f;
+        uq = 0;
         for (int l = 0; l < L; ++l) acc[l] = 0;
     }
-    // u += r y_t/m_t (fp32); acc_l += r W16[t][l]  (r < 256, W16 < 2^16: products < 2^24, sums < 2^29)
+    // uq += r F_t; acc_l += r W16[t][l]  (r < 256, W16 < 2^16: products < 2^24, sums < 2^29)
     template <typename Tab>
     OZ2_HD void add(Tab tb, int t, uint32_t r) {
-        u = __builtin_fmaf(static_cast<float>(r), tb->frac[t], u);
+        uq += mul24(r, tb->F16[t]);
         for (int l = 0; l < L; ++l) acc[l] += mul24(r, tb->W16[t][l]);
     }
     // moduli 2i and 2i+1 at once (r1 = 0 past s): one dot2 per limb with the paired table
     template <typename Tab>
     OZ2_HD void add2(Tab tb, int i, uint32_t r0, uint32_t r1) {
-        u = __builtin_fmaf(static_cast<float>(r0), tb->frac[2 * i], u);
-        u = __builtin_fmaf(static_cast<float>(r1), tb->frac[2 * i + 1], u);
         const uint32_t rr = r0 | (r1 << 16);
+        uq = udot2(rr, tb->F16p[i], uq);
         for (int l = 0; l < L; ++l) acc[l] = udot2(rr, tb->W16p[i][l], acc[l]);
     }
-    // RNE(C') * 2^-shift with C' = sum r_t W_t - q M, q = rint(u)
+    // RNE(C') * 2^-shift with C' = sum r_t W_t - q M
     template <typename Tab>
     OZ2_HD double finish(Tab tb, int shift) const {
-        const int32_t q = static_cast<int32_t>(__builtin_rintf(u));
+        const int32_t q = static_cast<int32_t>((uq + 0x8000u) >> 16);
         uint32_t w[NL];
         int32_t carry = 0;
         for (int l = 0; l < L; ++l) {

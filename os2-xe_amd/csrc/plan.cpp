@@ -160,13 +160,15 @@ void make_tabsets(TabSet* out) {
             tb.h2[t] = 2 * h;
             for (int j = 0; j < 16; ++j) tb.c8[t][j >> 2] |= static_cast<uint32_t>(pow2_mod(8 * j, m)) << (8 * (j & 3));
             tb.c30[t] = m == 256 ? 0 : m * static_cast<uint32_t>(((uint64_t{1} << 30) + m - 1) / m);
-            tb.frac[t] = static_cast<float>(static_cast<double>(p.y[t]) / m);
+            tb.F16[t] = static_cast<uint32_t>((static_cast<uint64_t>(p.y[t]) * (1u << 17) + m) / (2u * m));
             for (int l = 0; l < tb.L16; ++l) tb.W16[t][l] = (p.W[t].word32(l / 2) >> (16 * (l % 2))) & 0xFFFFu;
         }
         for (int l = 0; l < tb.L16; ++l) tb.M16[l] = (p.M.word32(l / 2) >> (16 * (l % 2))) & 0xFFFFu;
-        for (int t = 0; t < s; t += 2)
+        for (int t = 0; t < s; t += 2) {
             for (int l = 0; l < tb.L16; ++l)
                 tb.W16p[t / 2][l] = tb.W16[t][l] | (t + 1 < s ? tb.W16[t + 1][l] << 16 : 0u);
+            tb.F16p[t / 2] = tb.F16[t] | (t + 1 < s ? tb.F16[t + 1] << 16 : 0u);
+        }
     }
 }
 

@@ -94,7 +94,8 @@ struct TabSet {
     uint32_t c8[kMaxModuli][4];             // bytes 2^(8 j) mod m, j = 0..15, four to a word (low first)
     uint32_t c30[kMaxModuli];               // m * ceil(2^30 / m) (GEMM residue epilogue offset; 0 for m = 256)
     // CRT (hpp:222-236)
-    float frac[kMaxModuli];                 // fp32(y_t / m_t)
+    uint32_t F16[kMaxModuli];               // round(2^16 y_t / m_t): fixed-point quotient weights
+    uint32_t F16p[kMaxModuli / 2];          // F16[2i] | F16[2i+1] << 16 (0 past s)
     uint32_t W16[kMaxModuli][2 * kMaxLimbs];  // 16-bit limbs of W_t
     uint32_t W16p[kMaxModuli / 2][2 * kMaxLimbs];  // W16[2i][l] | W16[2i+1][l] << 16 (0 past s)
     uint32_t M16[2 * kMaxLimbs];            // 16-bit limbs of M

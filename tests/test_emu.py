@@ -169,6 +169,26 @@ def test_crt_random_vs_oracle(s, rng):
     assert np.array_equal(out.view(np.uint64), want.view(np.uint64))
 
 
+def test_fixed_point_quotient_margin():
+    # The CRT takes q = round(sum_t r_t F_t / 2^16), F_t = round(2^16 y_t / m_t).  Its error
+    # against the exact sum_t r_t y_t / m_t, at most sum_t (m_t - 1) |F_t / 2^16 - y_t / m_t|,
+    # must leave the 1/4 margin of |C'| <= M/4 short of 1/2 for every moduli set; and the
+    # residues of the extreme C' = +-M/4 (and of -1, every residue m_t - 1) give the exact q.
+    from fractions import Fraction
+    for s in range(2, 25):
+        plan = O.make_plan(s, 16)
+        ms, ys = O.moduli(s), [int(plan.y[t]) for t in range(s)]
+        F = [(y * 2 ** 17 + m) // (2 * m) for y, m in zip(ys, ms)]
+        err = sum(Fraction(m - 1) * abs(Fraction(f, 2 ** 16) - Fraction(y, m)) for f, y, m in zip(F, ys, ms))
+        assert err < Fraction(1, 4) - Fraction(1, 100), (s, float(err))
+        M = plan.M_int
+        for x in (M // 4, -(M // 4), -1, 0, 1):
+            r = [x % m for m in ms]
+            exact = sum(Fraction(ri * y, m) for ri, y, m in zip(r, ys, ms))
+            q = (sum(ri * f for ri, f in zip(r, F)) + 2 ** 15) >> 16
+            assert q == round(exact) and abs(exact - q) <= Fraction(1, 4), (s, x)
+
+
 def test_exponent_rule():
     for b, tk in ((0, 30), (0x7FF0000000000000, 30), (0x7FF8000000000000, 5), (1, 55), (0x3FF0000000000000, 55),
                   (0x000FFFFFFFFFFFFF, 55), (0x7FEFFFFFFFFFFFFF, 100)):
