@@ -34,8 +34,8 @@ HBM_PEAK_GBS = 8000.0    # MI355X HBM3E, 8 TB/s
 # A bench run cannot count its own traffic (the counters need rocprofv3 around the process), so the
 # line quotes these with their source; the GEMM's fetch (its L2 misses) differs by a few % between boxes.
 MEASURED_GEMM_TRAFFIC = {
-    ("dgemm", 8192, 8192, 8192, 16): (1.4467e10, "profiles/dgemm8192_s16_hbm_traffic.txt (one MI355X, 2026-12-03)"),
-    ("zgemm", 4096, 4096, 4096, 16): (5.6766e9, "profiles/zgemm4096_s16_hbm_traffic.txt (one MI355X, 2026-12-03)"),
+    ("dgemm", 8192, 8192, 8192, 16): (1.4405e10, "profiles/dgemm8192_s16_hbm_traffic.txt (one MI355X, 2026-12-03)"),
+    ("zgemm", 4096, 4096, 4096, 16): (5.6388e9, "profiles/zgemm4096_s16_hbm_traffic.txt (one MI355X, 2026-12-03)"),
 }
 
 
