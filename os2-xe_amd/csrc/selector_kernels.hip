@@ -148,10 +148,21 @@ __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __rest
 #pragma unroll
         for (int b = 0; b < kBins; ++b) lc[b] += b == bin ? 1u : 0u;
     }
+    // the wave's counts summed across its lanes (each at most 64 x 128 elements; the gap sum
+    // at most 64 x 128 x 2047 < 2^32), then one LDS atomic per count and wave
+    uint32_t gs = static_cast<uint32_t>(gsum);
 #pragma unroll
-    for (int b = 0; b <= kBins; ++b)
-        if (lc[b]) atomicAdd(&bcnt[b], static_cast<unsigned long long>(lc[b]));
-    if (gsum) atomicAdd(&bcnt[kBins + 1], gsum);
+    for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+        for (int b = 0; b <= kBins; ++b) lc[b] += __shfl_xor(lc[b], o, 64);
+        gs += __shfl_xor(gs, o, 64);
+    }
+    if ((t & 63) == 0) {
+#pragma unroll
+        for (int b = 0; b <= kBins; ++b)
+            if (lc[b]) atomicAdd(&bcnt[b], static_cast<unsigned long long>(lc[b]));
+        if (gs) atomicAdd(&bcnt[kBins + 1], static_cast<unsigned long long>(gs));
+    }
     atomicMax(&rgmax[rl], gmax);
     __syncthreads();
     SideAcc& a = acc[2 * p + side];
