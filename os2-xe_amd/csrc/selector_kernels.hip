@@ -61,15 +61,17 @@ __device__ __forceinline__ int64_t sampled_pos(int j, int64_t k) {
 
 // One block per group of 16 consecutive sampled rows of op(A) or columns of op(B), all
 // their sampled positions.  Lanes run along memory: along k for rows contiguous in memory
-// (op(A) = T, op(B) = N; thread t: row t >> 4, positions (t & 15) + 16 i), along the rows
-// for strided ones (op(A) = N, op(B) = T; row t & 15, positions (t >> 4) + 16 i), where
+// (op(A) = T, op(B) = N; thread t: row t >> 5, positions (t & 31) + 32 i), along the rows
+// for strided ones (op(A) = N, op(B) = T; row t & 15, positions (t >> 4) + 32 i), where
 // sampled rows less than 16 apart share the lines a load fetches.  Pass 1 keeps the
 // exponent field of every sampled element (0xFFFF for a zero) in LDS, position-major,
 // and finds each row's largest; pass 2 bins the gaps to it.  The block's counts go to
 // the (problem, side) accumulator in one set of atomics (one set per row took as long as
 // all the loads).
 constexpr int kSelGroup = 16;
-__global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __restrict__ probs, int nprob,
+constexpr int kSelThreads = 512;                    // 32 position lanes per row
+constexpr int kSelLanes = kSelThreads / kSelGroup;
+__global__ void __launch_bounds__(kSelThreads) sel_stats_kernel(const SelProblem* __restrict__ probs, int nprob,
                                                         SideAcc* __restrict__ acc) {
     constexpr int groups = kSelRows / kSelGroup;
     const int p = blockIdx.x / (2 * groups);
@@ -83,7 +85,7 @@ __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __rest
     if (g * kSelGroup >= nr || np <= 0) return;
     const bool contig = side == 0 ? q.ta != 0 : q.tb == 0;  // k runs along memory
     const int t = static_cast<int>(threadIdx.x);
-    const int rl = contig ? t >> 4 : t & (kSelGroup - 1), kl = contig ? t & 15 : t >> 4;
+    const int rl = contig ? t / kSelLanes : t % kSelGroup, kl = contig ? t % kSelLanes : t / kSelGroup;
     const int r = g * kSelGroup + rl;
     const bool rowok = r < nr;
     const int64_t i = rowok ? static_cast<int64_t>(r) * rows / nr : 0;  // absent rows read row 0, unused
@@ -106,17 +108,17 @@ __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __rest
     // pass 1: exponents into LDS and row maxima; 16 loads in flight per step, all at j < np
     constexpr int U = 16;
     unsigned int emax = 0;
-    for (int j0 = kl; j0 < np; j0 += 16 * U) {
+    for (int j0 = kl; j0 < np; j0 += kSelLanes * U) {
         uint64_t v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int j = min(j0 + 16 * u, np - 1);
+            const int j = min(j0 + kSelLanes * u, np - 1);
             v[u] = static_cast<uint64_t>(__double_as_longlong(row[sampled_pos(j, kk) * step_k])) &
                    0x7FFFFFFFFFFFFFFFull;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int j = j0 + 16 * u;
+            const int j = j0 + kSelLanes * u;
             if (j < np) {
                 const unsigned int e = static_cast<unsigned int>(v[u] >> 52);
                 ex[j][rl] = v[u] == 0 ? uint16_t{0xFFFF} : static_cast<uint16_t>(e);
@@ -135,7 +137,7 @@ __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __rest
     unsigned int lc[kBins + 1];
 #pragma unroll
     for (int b = 0; b <= kBins; ++b) lc[b] = 0;
-    for (int j = kl; rowok && j < np; j += 16) {
+    for (int j = kl; rowok && j < np; j += kSelLanes) {
         const unsigned int e = ex[j][rl];
         if (e == 0xFFFF) {
             ++lc[kBins];
@@ -148,8 +150,8 @@ __global__ void __launch_bounds__(256) sel_stats_kernel(const SelProblem* __rest
 #pragma unroll
         for (int b = 0; b < kBins; ++b) lc[b] += b == bin ? 1u : 0u;
     }
-    // the wave's counts summed across its lanes (each at most 64 x 128 elements; the gap sum
-    // at most 64 x 128 x 2047 < 2^32), then one LDS atomic per count and wave
+    // the wave's counts summed across its lanes (each at most 64 x 64 elements; the gap sum
+    // at most 64 x 64 x 2047 < 2^32), then one LDS atomic per count and wave
     uint32_t gs = static_cast<uint32_t>(gsum);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
@@ -380,8 +382,8 @@ void sel_features(hipStream_t st, const SelProblem* probs, int nprob, void* scra
     if (nprob <= 0) return;
     SideAcc* acc = static_cast<SideAcc*>(scratch);
     hipLaunchKernelGGL(sel_clear_kernel, dim3(64), dim3(256), 0, st, acc, 2 * nprob);
-    hipLaunchKernelGGL(sel_stats_kernel, dim3(static_cast<unsigned>(nprob) * 2 * (kSelRows / kSelGroup)), dim3(256),
-                       0, st, probs, nprob, acc);
+    hipLaunchKernelGGL(sel_stats_kernel, dim3(static_cast<unsigned>(nprob) * 2 * (kSelRows / kSelGroup)),
+                       dim3(kSelThreads), 0, st, probs, nprob, acc);
     hipLaunchKernelGGL(sel_finalize_kernel, dim3(static_cast<unsigned>(nprob)), dim3(kSelRows), 0, st, probs, nprob,
                        acc, feat);
 }
