@@ -188,7 +188,9 @@ def test_unit_gemm_against_int64_brute_force(tile):
     # tile and to kp), every residue byte against (sum of int64 products) mod m_t
     rng = np.random.default_rng(tile)
     tm, tn = (256, 128) if tile == 256128 else (tile, tile)
-    for m, n, k, s in ((tm + 37, 2 * tn - 5, 300, 16), (tm - 3, tn + 1, 1, 5), (2 * tm + 1, 3 * tn - 1, 2500, 24)):
+    cases = ((tm + 37, 2 * tn - 5, 300, 16), (tm - 3, tn + 1, 1, 5), (2 * tm + 1, 3 * tn - 1, 2500, 24),
+             (tm + 5, tn - 7, 256, 7))
+    for m, n, k, s in cases:
         mpad, npad = -(-m // tm) * tm, -(-n // tn) * tn
         kp = -(-k // 128) * 128
         ldr = mpad + 16
@@ -197,9 +199,12 @@ def test_unit_gemm_against_int64_brute_force(tile):
         b = np.zeros((s, npad, kp), dtype=np.int8)
         a[:, :m, :k] = rng.integers(-128, 128, (s, m, k))
         b[:, :n, :k] = rng.integers(-128, 128, (s, n, k))
-        # the first planes at the extremes: all -128 (largest |sum|) and alternating signs
-        a[0, :m, :k] = -128
-        b[0, :n, :k] = -128
+        # the first planes at the extremes: all -128 (the largest sum; for k <= 256 the top of
+        # the epilogue's short form), and -128 against 127 (its most negative sum)
+        a[0:2, :m, :k] = -128
+        b[0:2, :n, :k] = -128
+        a[2, :m, :k] = -128
+        b[2, :n, :k] = 127
         ia, ib = plane_index(mpad, kp).reshape(-1), plane_index(npad, kp).reshape(-1)
         pa = np.zeros((s, mpad * kp), dtype=np.int8)
         pb = np.zeros((s, npad * kp), dtype=np.int8)
