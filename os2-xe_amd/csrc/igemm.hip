@@ -168,8 +168,14 @@ __device__ __forceinline__ void int32_epilogue(const GemmArgs& p, v16i (&acc)[FM
 }
 
 // one BM x BN macro tile of one plane: k loop and residue (or int32) epilogue; smem holds NS stages
-template <int BM, int BN, int NS, bool I32 = false>
-__device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uint32_t tm, uint32_t tn, v4i* smem) {
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+// after_loop runs once every DMA of the tile has landed (vm_wait(0)) and before the epilogue
+template <int BM, int BN, int NS, bool I32 = false, typename Hook = NoHook>
+__device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uint32_t tm, uint32_t tn, v4i* smem,
+                                          const Hook& after_loop = Hook{}) {
     constexpr int WTM = BM / 2, WTN = BN / 2;
     constexpr int FM = WTM / 32, FN = WTN / 32;
     constexpr int LA = BM / 64, LB = BN / 64;  // 1 KB DMAs per wave per stage
@@ -281,6 +287,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& p, uint32_t plane, uin
 #undef OZ2_HALF
 #undef OZ2_WAIT
     __builtin_amdgcn_s_waitcnt(vm_wait(0));  // no DMA may still target LDS
+    after_loop();
     if constexpr (I32) {
         int32_epilogue<FM, FN>(p, acc, plane, static_cast<int64_t>(tm) * BM + wm * WTM,
                                static_cast<int64_t>(tn) * BN + wn * WTN, lane);
@@ -316,7 +323,9 @@ __global__ void __launch_bounds__(256, MINB) igemm_int32_kernel(GemmArgs p) {
 // form (workgroup b runs on XCD b % 8), so the tiles that share A and B bands meet in
 // one L2; each range has a counter of its own (counter[0..7]), and a workgroup whose
 // range is done takes work from the next ranges in turn.  Every counter only grows, so
-// each workgroup leaves after at most 8 empty draws.
+// each workgroup leaves after at most 8 empty draws.  Every drawn index is examined at the
+// top of the loop: one below its range's end is a tile, processed once (the counters hand
+// out each index once), one past it moves the workgroup to the next range.
 template <int BM, int BN, int MINB, int NS>
 __global__ void __launch_bounds__(256, MINB) igemm_grouped_kernel(const GemmArgs* __restrict__ items,
                                                                   const uint32_t* __restrict__ start,
@@ -329,23 +338,44 @@ __global__ void __launch_bounds__(256, MINB) igemm_grouped_kernel(const GemmArgs
     uint32_t& next = *reinterpret_cast<uint32_t*>(smem + TileShape<BM, BN, NS>::SMEM / 16);
     const uint32_t x0 = blockIdx.x & 7;
     uint32_t tried = 0;  // ranges found empty, from x0 on
+    auto range = [&](uint32_t x, uint32_t& lo, uint32_t& hi) {
+        lo = static_cast<uint32_t>(static_cast<uint64_t>(total) * x / 8);
+        hi = static_cast<uint32_t>(static_cast<uint64_t>(total) * (x + 1) / 8);
+    };
+    // the draw for a tile is issued one tile ahead, when the previous tile starts, and its
+    // result is parked in LDS once that tile's k-loop is done: the draw's return is then waited
+    // for before the previous tile's residue stores are issued, not after them (a wave's
+    // vector memory counter covers stores as well as loads, so a draw issued after the stores
+    // would wait for their write acknowledgements)
+    uint32_t x = x0, lo, hi;
+    range(x, lo, hi);
+    if (threadIdx.x == 0) next = lo + atomicAdd(&counter[x], 1u);
+    __syncthreads();
+    uint32_t w = __builtin_amdgcn_readfirstlane(next);
+    __syncthreads();
     for (;;) {
-        const uint32_t x = (x0 + tried) & 7;
-        const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(total) * x / 8);
-        const uint32_t hi = static_cast<uint32_t>(static_cast<uint64_t>(total) * (x + 1) / 8);
-        if (threadIdx.x == 0) next = lo + atomicAdd(&counter[x], 1u);
-        __syncthreads();
-        const uint32_t w = __builtin_amdgcn_readfirstlane(next);  // uniform: item fields stay in SGPRs
-        __syncthreads();  // also: the previous tile's epilogue is done with LDS
-        if (w >= hi) {
+        if (w >= hi) {  // this range is done: draw from the next one
             if (++tried == 8) break;
+            x = (x0 + tried) & 7;
+            range(x, lo, hi);
+            if (threadIdx.x == 0) next = lo + atomicAdd(&counter[x], 1u);
+            __syncthreads();
+            w = __builtin_amdgcn_readfirstlane(next);
+            __syncthreads();
             continue;
         }
+        uint32_t ticket = 0;
+        if (threadIdx.x == 0) ticket = atomicAdd(&counter[x], 1u);  // the tile after this one
         const uint32_t it = owner[w];
         const GemmArgs& p = items[it];
         uint32_t plane, tm, tn;
         tile_coords(p, w - start[it], plane, tm, tn);
-        gemm_tile<BM, BN, NS>(p, plane, tm, tn, smem);
+        gemm_tile<BM, BN, NS>(p, plane, tm, tn, smem, [&] {
+            if (threadIdx.x == 0) next = lo + ticket;
+        });
+        __syncthreads();  // the epilogue is done with LDS; next is visible
+        w = __builtin_amdgcn_readfirstlane(next);
+        __syncthreads();
     }
 }
 
